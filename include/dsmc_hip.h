@@ -1,0 +1,258 @@
+/*
+ * dsmc_hip.h - C ABI of the MI355X-native test-particle DSMC path.
+ *
+ * Two groups of entry points:
+ *
+ *  1. Drop-in replacements for dsmc_suite's drivers and file I/O.  Same names,
+ *     same signatures, same file side effects (`<prefix>_den.dat`, ...):
+ *       run_ballistic  replaces src/dsmc/ballistic.c:35-36 (declared extern at
+ *                      src/main.c:18-19, called at src/main.c:147,194)
+ *       run_advanced   replaces src/dsmc/advanced.c:119-121 (declared extern at
+ *                      src/main.c:20-23, called at src/main.c:150,202)
+ *       read_ / write_*_dat  replace include/fileio.h:8-44 (src/common/fileio.c)
+ *     `nthreads` keeps its meaning of host threads; the device is implicit
+ *     (HIP device 0 unless DSMC_DEVICE is set in the environment).
+ *
+ *  2. The parameterised in-memory API (dsmc_run_*), the multi-run batch API and
+ *     the serving API.  They return an int status (0 = ok, <0 = dsmc_status).
+ *
+ * All pointers are plain host pointers owned by the caller unless stated.
+ * No torch or HIP types appear in any signature.
+ */
+#ifndef DSMC_HIP_H
+#define DSMC_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+enum dsmc_status {
+    DSMC_OK = 0,
+    DSMC_ERR_ARG = -1,        /* invalid argument (sizes, NULL where required) */
+    DSMC_ERR_IO = -2,         /* file missing, short or malformed */
+    DSMC_ERR_DEVICE = -3,     /* no usable HIP device / kernel image */
+    DSMC_ERR_NOMEM = -4,      /* device or host allocation failed */
+    DSMC_ERR_BUDGET = -5,     /* request does not fit the serving HBM budget */
+    DSMC_ERR_CANCELLED = -6,  /* serving request cancelled before completion */
+    DSMC_ERR_BUSY = -7        /* serving queue full / result not ready */
+};
+
+/* ------------------------------------------------------------------ modes */
+enum dsmc_mode {
+    /* Reproduces dsmc_suite's run_ballistic / run_advanced state machines,
+     * quirks included (uniform-angle launch, velocity table, launch-cell frozen
+     * scan, polar deposit, fixed dayside fd < ffd/2, W-decay photolysis). */
+    DSMC_MODE_COMPAT = 0,
+    /* The exosphere model of the benchmark configurations: icosphere surface,
+     * sublimation-flux-weighted launch, flux-Maxwellian (Lambertian) emission at
+     * the local face temperature, freeze-out on impact below t_freeze,
+     * Monte-Carlo photolysis on the sunlit half-space, per-particle step cap. */
+    DSMC_MODE_PHYSICAL = 1
+};
+
+enum dsmc_fate {
+    DSMC_FATE_NONE = 0,
+    DSMC_FATE_ESCAPED = 1,     /* reached max_height, or launch speed >= v_esc (compat) */
+    DSMC_FATE_FROZEN = 2,      /* frozen on a cold surface */
+    DSMC_FATE_PHOTOLYSED = 3,  /* photodissociated (physical mode) */
+    DSMC_FATE_DEPLETED = 4,    /* weight fell below w_min (compat: photolysis + polar loss) */
+    DSMC_FATE_CAPPED = 5       /* per-particle step cap reached (physical mode) */
+};
+
+enum dsmc_rng_stream {
+    DSMC_STREAM_BALLISTIC = 1,
+    DSMC_STREAM_ADVANCED = 2
+};
+
+/* ------------------------------------------------------------------ params */
+/* One run's scalars.  Derived quantities are computed from these exactly the way
+ * the reference computes them (GM = G*M, v_esc = sqrt(2*G*M/R), molecule mass =
+ * mmass/mole, dt_rot = period/ffd), so compat runs reproduce its arithmetic. */
+typedef struct dsmc_params {
+    int mode;                 /* enum dsmc_mode */
+    int stream;               /* enum dsmc_rng_stream (Philox counter word 3) */
+    uint64_t seed;            /* Philox key */
+    int64_t nparticles;       /* test particles in this run */
+
+    /* body and species (include/constants.h:5-35 for the Ceres/H2O preset) */
+    double G, M, radius;
+    double k_B, mmass, mole;
+    double lifetime;          /* photolysis lifetime tau [s] */
+    double period;            /* rotation period P [s] */
+    double alpha;             /* sticking coefficient (compat velocity_sticking) */
+    double v_esc;             /* escape speed used by the compat launch test */
+
+    /* integration */
+    double dt;                /* Verlet step h [s] */
+    int64_t max_hop_steps;    /* per-hop step limit (reference: 1e7) */
+    int64_t max_steps;        /* per-particle step cap (physical), 0 = none */
+
+    /* thresholds */
+    double t_freeze;          /* compat: launch-cell T < t_freeze freezes; physical: impact T */
+    double w_min;             /* particle ends when W < w_min (reference 0.01) */
+    double subsolar_lon;      /* physical: longitude of the subsolar point [rad] */
+
+    /* grid */
+    int ftd, ffd, hd;         /* latitude bins, longitude bins, radial shells */
+    int exp_grid;             /* 0: uniform shells of unih, 1: exponential (alpha2) */
+    double unih, alpha2, max_height;
+
+    /* velocity records (compat ballistic v_record, reference ballistic.c:77-82) */
+    int record_cap_cell;      /* per (td, ht) cap, reference 200; 0 disables */
+    int64_t record_cap_total; /* global cap, reference 100*ftd*hd */
+
+    /* collisions (advanced) */
+    double cross_section;     /* sigma [m^2] */
+    int fit_degree;           /* polynomial CDF degree (reference POLY_DEGREE = 3) */
+    int max_per_cell;         /* speeds kept per cell for the fit (reference 50) */
+    int min_fit_samples;      /* fit only cells with nn > this (reference 8) */
+    double density_scale;     /* multiplies the phase-1 density read as copy_atm */
+
+    /* physical-mode emission */
+    double sublimation_scale; /* multiplies face sublimation weights (cfg3: x10) */
+} dsmc_params;
+
+/* Presets.  COMPAT_BALLISTIC / COMPAT_ADVANCED reproduce the reference CLI
+ * (ballistic.c:49-67, advanced.c:131-152; ftd/ffd come from the T file).
+ * CERES_PHYSICAL is the benchmark's Ceres/H2O ballistic configuration. */
+enum dsmc_preset {
+    DSMC_PRESET_COMPAT_BALLISTIC = 0,
+    DSMC_PRESET_COMPAT_ADVANCED = 1,
+    DSMC_PRESET_CERES_PHYSICAL = 2,
+    DSMC_PRESET_CERES_PHYSICAL_COLL = 3
+};
+int dsmc_params_preset(dsmc_params *p, int preset, int ftd, int ffd);
+
+/* ------------------------------------------------------------------ surface */
+/* Compat runs read a lat/lon temperature field [ftd][ffd].  Physical runs use an
+ * icosphere: faces of all subdivision levels are stored level by level (20, 80,
+ * 320, ... faces); the children of face f of level l are faces 4f..4f+3 of level
+ * l+1 (the subdivision order of scripts/vesta_raytrace_gif.py:35-102). */
+typedef struct dsmc_surface {
+    int ftd, ffd;
+    const double *T_latlon;   /* [ftd*ffd], compat */
+
+    int levels;               /* subdivision levels L; finest level has 20*4^L faces */
+    int nvert;
+    const double *vert;       /* [nvert*3], unit vectors */
+    const int32_t *faces;     /* [sum_l 20*4^l][3] vertex indices, all levels */
+    const double *face_T;     /* [20*4^L] temperature of each finest face [K] */
+    const double *face_w;     /* [20*4^L] launch weight of each finest face */
+} dsmc_surface;
+
+/* Builds an icosphere with `levels` subdivisions into caller buffers sized by
+ * dsmc_icosphere_sizes, with the synthetic temperature
+ * T = t_night + t_amp * max(cos(sza), 0)^0.25 and Antoine/Hertz-Knudsen
+ * sublimation weights (face area x flux).  Returns 0 or DSMC_ERR_ARG. */
+int dsmc_icosphere_sizes(int levels, int *nvert, int *nface_all, int *nface_fine);
+int dsmc_icosphere_build(int levels, double subsolar_lon, double t_night, double t_amp,
+                         double molecule_mass, double k_B,
+                         double *vert, int32_t *faces, double *face_T, double *face_w);
+/* Rasterises the face temperatures to a lat/lon grid (cell centre -> face). */
+int dsmc_icosphere_rasterise(const dsmc_surface *s, int ftd, int ffd, double *T_latlon);
+
+/* ------------------------------------------------------------------ results */
+/* Every pointer may be NULL (that output is then not produced). */
+typedef struct dsmc_result {
+    double *density;          /* [hd*ftd*ffd], [ht][td][fd], sum of W per step */
+    double *horizon;          /* [ftd*ffd] frozen / polar deposits */
+    double *vmom;             /* [hd*ftd*ffd*3] sum of velocity per deposit (physical) */
+    int64_t *vcount;          /* [hd*ftd*ffd] deposits per cell (physical) */
+    double *ttime;            /* [rows*n], rows 5 (ballistic) or 9 (advanced) */
+    double *end_state;        /* [n*6] final position and velocity */
+    int64_t *steps;           /* [n] Verlet steps taken by each particle */
+    int32_t *fate;            /* [n] enum dsmc_fate */
+    int32_t *hops;            /* [n] launches per particle */
+    /* compat ballistic velocity records, capacity record_cap_total */
+    double *vrec;             /* [cap*3] */
+    int32_t *vrec_td, *vrec_fd, *vrec_ht;
+    int64_t vrec_n;           /* out: records written */
+    /* out: counters */
+    int64_t n_escaped, n_frozen, n_photolysed, n_depleted, n_capped;
+    int64_t n_collisions, total_steps;
+    double elapsed_ms;        /* out: device time of the run */
+} dsmc_result;
+
+/* Phase-2 inputs (advanced): the phase-1 density read as copy_atm and the
+ * velocity records.  Both optional, as in advanced.c:160-209. */
+typedef struct dsmc_phase2_input {
+    const double *copy_atm;   /* [atm_hd*atm_ftd*atm_ffd] or NULL (no collisions) */
+    int atm_ftd, atm_ffd, atm_hd;
+    const double *vrec;       /* [nrec*3] or NULL (no fits) */
+    const int32_t *vrec_td, *vrec_fd, *vrec_ht;
+    int64_t nrec;
+} dsmc_phase2_input;
+
+/* Single runs on the current device.  Inputs stay in host memory; the call
+ * uploads them, runs, and copies results back. */
+int dsmc_run_ballistic(const dsmc_params *p, const dsmc_surface *s, dsmc_result *r);
+int dsmc_run_advanced(const dsmc_params *p, const dsmc_surface *s,
+                      const dsmc_phase2_input *in, dsmc_result *r);
+
+/* Per-cell polynomial CDF fits (advanced.c:32-117 + polyfit.c:10-101) computed on
+ * the device from velocity records: outputs [ftd*ffd*hd] cells in the
+ * reference's [td][fd][ht] order, coeffs[cell*(degree+1)], highest power first. */
+int dsmc_build_fits(const dsmc_params *p, const dsmc_phase2_input *in,
+                    int32_t *count_num, double *coeffs, double *min_v, double *max_v,
+                    int32_t *valid);
+
+/* ------------------------------------------------------------------ batch */
+/* Runs `nruns` independent runs in one persistent launch.  params[i],
+ * surfaces[i] and results[i] describe run i.  `tune` selects the launch shape:
+ * NULL = autotuned (cached per device and workload class). */
+typedef struct dsmc_launch_shape {
+    int block;                /* threads per block (multiple of 64) */
+    int blocks_per_cu;        /* resident blocks per CU */
+    int relaunch_batch;       /* lanes that must land before a wave relaunches them */
+} dsmc_launch_shape;
+int dsmc_run_batch(int nruns, const dsmc_params *params, const dsmc_surface *surfaces,
+                   dsmc_result *results, const dsmc_launch_shape *tune);
+int dsmc_autotune(const dsmc_params *p, const dsmc_surface *s, dsmc_launch_shape *out);
+
+/* ------------------------------------------------------------------ serving */
+/* A continuous-batching server: requests are admitted while their HBM footprint
+ * (particles x 96 B + grid) fits the budget, run together with whatever is
+ * already on the device, and complete independently. */
+typedef struct dsmc_server dsmc_server;
+dsmc_server *dsmc_server_create(size_t hbm_budget_bytes, int max_inflight);
+int dsmc_server_submit(dsmc_server *sv, const dsmc_params *p, const dsmc_surface *s,
+                       dsmc_result *r, uint64_t *ticket);
+int dsmc_server_poll(dsmc_server *sv, uint64_t ticket);   /* 1 done, 0 pending, <0 error */
+int dsmc_server_wait(dsmc_server *sv, uint64_t ticket);
+int dsmc_server_cancel(dsmc_server *sv, uint64_t ticket);
+int dsmc_server_step(dsmc_server *sv);                     /* admit + advance, non-blocking */
+void dsmc_server_destroy(dsmc_server *sv);
+
+/* ------------------------------------------------------------------ drop-in */
+void run_ballistic(int totaln, const char *temp_file, const char *output_prefix,
+                   int nthreads);
+void run_advanced(int totaln, const char *temp_file, const char *prev_den_file,
+                  const char *prev_v_record_file, const char *output_prefix,
+                  int nthreads);
+
+/* include/fileio.h:8-44, same semantics and printf formats */
+double *read_density_dat(const char *filename, int *ftd, int *ffd, int *hd, int has_hd);
+int write_density_dat(const char *filename, const double *den, int ftd, int ffd, int hd);
+double *read_temperature_dat(const char *filename, int *ftd, int *ffd);
+double *read_temperature_time_dat(const char *filename, int *ftd, int *ffd);
+int write_ttime_dat(const char *filename, const double *ttime, int nrows, int totaln);
+int write_v_record_dat(const char *filename, const double *v_record, const int *td_list,
+                       const int *fd_list, const int *ht_list, int nrecords);
+int read_v_record_dat(const char *filename, double **v_record, int **td_list,
+                      int **fd_list, int **ht_list, int *nrecords);
+double *read_flux_dat(const char *filename, int *nface, int *tt);
+int write_flux_dat(const char *filename, const double *F, int nface, int tt);
+int write_horizon_dat(const char *filename, const double *horizon, int ftd, int ffd);
+
+/* ------------------------------------------------------------------ misc */
+const char *dsmc_version(void);
+int dsmc_device_info(char *buf, size_t len);   /* "gfx950, 256 CUs, ..." */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DSMC_HIP_H */
