@@ -182,6 +182,18 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.dsmc_server_step.argtypes = [C.c_void_p]
     lib.dsmc_server_destroy.argtypes = [C.c_void_p]
     lib.dsmc_server_destroy.restype = None
+    lib.dsmc_run_create.argtypes = [P, S, R, C.POINTER(C.c_int)]
+    lib.dsmc_run_create.restype = C.c_void_p
+    lib.dsmc_run_launch.argtypes = [C.c_void_p, C.POINTER(LaunchShape)]
+    lib.dsmc_run_sync.argtypes = [C.c_void_p]
+    lib.dsmc_run_kernel_ms.argtypes = [C.c_void_p]
+    lib.dsmc_run_kernel_ms.restype = C.c_double
+    lib.dsmc_run_collect.argtypes = [C.c_void_p, R]
+    lib.dsmc_run_destroy.argtypes = [C.c_void_p]
+    lib.dsmc_run_destroy.restype = None
+    lib.dsmc_verlet_steps.argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, _dp]
+    lib.dsmc_elementary.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int]
     lib.run_ballistic.argtypes = [C.c_int, C.c_char_p, C.c_char_p, C.c_int]
     lib.run_ballistic.restype = None
     lib.run_advanced.argtypes = [C.c_int, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]

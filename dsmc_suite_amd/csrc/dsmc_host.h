@@ -1,0 +1,46 @@
+/* dsmc_host.h - internal host-side types shared by the library's translation units. */
+#ifndef DSMC_HOST_H
+#define DSMC_HOST_H
+
+#include <hip/hip_runtime.h>
+#include "../../include/dsmc_hip.h"
+#include "dsmc_tables.h"
+
+namespace dsmc {
+
+/* Owning device allocation. */
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf();
+    void release();
+    int alloc(size_t n);
+    int upload(const void *src, size_t n, hipStream_t s);
+    int zeros(size_t n, hipStream_t s);
+};
+
+/* Device-side state of one run. */
+struct RunDev {
+    DsmcEnv E;
+    HostTables T;
+    DevBuf tq, hr, cs, vt_speeds, vt_count, Tsurf, cdf, enorm, vert, faces, face_T;
+    DevBuf density, horizon, vmom, vcount, ttime, end_state, steps, fate, hops;
+    DevBuf vrec, vrec_i, count_nn, ctl;
+    /* advanced */
+    DevBuf atm, count_num, coeffs, pmin, pmax, pvalid, speeds;
+};
+
+int ensure_device();
+int device_props(int *cus);
+int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r, int advanced,
+              RunDev *rd, hipStream_t st);
+int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st);
+int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st);
+dsmc_launch_shape default_shape(const DsmcEnv &E);
+
+}  // namespace dsmc
+
+#endif

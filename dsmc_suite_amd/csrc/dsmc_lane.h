@@ -1,0 +1,664 @@
+/*
+ * dsmc_lane.h - the per-particle state machine of the DSMC path, written once
+ * for the gfx950 kernels (one particle per lane) and for a test-only host build.
+ *
+ * A particle alternates between FLIGHT (one Stormer-Verlet step per call of
+ * lane_step) and the hop service (landing bookkeeping + relaunch, lane_service),
+ * so the kernels can run the cheap, uniform flight step for a whole wavefront and
+ * batch the divergent relaunch work.  Semantics follow the reference drivers:
+ *   compat ballistic  src/dsmc/ballistic.c:102-281
+ *   compat advanced   src/dsmc/advanced.c:241-450 (collision branch :369-398)
+ *   physical          the benchmark exosphere model (icosphere surface, flux-
+ *                     Maxwellian emission, freeze-out, Monte-Carlo photolysis)
+ * and every floating-point expression is evaluated in the reference's order with
+ * no contraction (the translation units are built with -ffp-contract=off), so a
+ * particle's trajectory is bit-identical to the oracle's device-math build.
+ *
+ * Per-step cell indices avoid transcendentals:
+ *   td  latitude: q = z/Ri compared against a host-built table tq[k] = smallest
+ *       q with ceil((asin(q)+pi/2)/pi*ftd)-1 >= k (libm asin, grid.c:28-39);
+ *   ht  radius: Ri compared against hr[k] = smallest Ri whose height index is
+ *       >= k (uniform: ballistic.c:211, exponential: grid.c:91-99 with libm log);
+ *   fd  longitude: kept while the point is farther than 1e-12 (relative) from
+ *       both sector boundaries (a sign test against the boundary directions),
+ *       otherwise recomputed as ceil(atan2(x,y)/2pi*ffd)-1 with dm_atan2.
+ */
+#ifndef DSMC_LANE_H
+#define DSMC_LANE_H
+
+#include <stdint.h>
+#include <math.h>
+#include "dsmc_math.h"
+#include "philox.h"
+
+#if defined(__HIPCC__)
+#define DSMC_HD __host__ __device__ inline
+#else
+#define DSMC_HD inline
+#endif
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+enum { LANE_IDLE = 0, LANE_FLIGHT = 1, LANE_LANDED = 2, LANE_DONE = 3 };
+enum { DSMC_VT_NT = 23, DSMC_VT_NV = 1000 };
+#define DSMC_FD_MARGIN 1e-12
+
+/* Everything a lane reads or writes; device pointers on the GPU. */
+struct DsmcEnv {
+    /* run scalars */
+    int mode, ftd, ffd, hd, exp_grid, advanced;
+    int64_t nparticles, max_hop_steps, max_steps;
+    uint64_t seed;
+    uint32_t stream;
+    double GM, R, h, c1, c2, decay, dt_rot, ve, max_height, w_min, t_freeze, alpha, period;
+    double mass, k_B, lifetime, sun_x, sun_y, omega, unih, alpha2;
+    /* tables */
+    const double *T;          /* [ftd*ffd] surface temperature (compat) */
+    const double *vt_speeds;  /* [23*1000] velocity table (velocity.c:6-47) */
+    const int32_t *vt_count;  /* [23] */
+    const double *tq;         /* [ftd+1] latitude thresholds on z/Ri */
+    const double *hr;         /* [hd+2] radial thresholds */
+    const double *cs;         /* [(ffd+1)*2] (cos, sin) of sector boundary k*2pi/ffd */
+    /* physical surface */
+    int levels, nfine;
+    const double *cdf;        /* [nfine] inclusive prefix sum of launch weights */
+    const double *face_T;     /* [nfine] */
+    const double *vert;       /* [nvert*3] */
+    const int32_t *faces;     /* all levels, [sum*3] */
+    const double *enorm;      /* all levels, [sum*9]: cross(a,b), cross(b,c), cross(c,a) */
+    int level_off[12];
+    /* advanced inputs */
+    const double *copy_atm;   /* scaled phase-1 density, [atm_total] */
+    int64_t atm_total;
+    double sigma;
+    int fit_deg;
+    const int32_t *count_num; /* [ftd*ffd*hd] cells [td][fd][ht] */
+    const double *coeffs;     /* [cells*(deg+1)] */
+    const double *pmin, *pmax;
+    const int32_t *pvalid;
+    /* outputs */
+    double *density, *horizon, *vmom;
+    unsigned long long *vcount;
+    double *ttime, *end_state;
+    int64_t *steps_out;
+    int32_t *fate_out, *hops_out;
+    int ttime_rows;
+    double *vrec;
+    int32_t *vrec_td, *vrec_fd, *vrec_ht;
+    int32_t *count_nn;        /* [ftd*hd] record counter per (td, ht) */
+    unsigned long long *rec_total;
+    int rec_cap_cell;
+    int64_t rec_cap_total;
+};
+
+struct DsmcLane {
+    double sx, sy, sz, vx, vy, vz, ox, oy, oz;
+    double Ri, W, time, flight_time, thi, fi;
+    double acc_w, acx, acy, acz;
+    double real_col_p, col_p, rest_time, col_lo;
+    double lsx, lsy, lsz;            /* advanced: position before a frozen-scan rotation */
+    int64_t steps, ii, n_day, n_photo, pid;
+    uint64_t k, blk;
+    uint32_t rb0, rb1, rb2, rb3;
+    int phase, fate, hops, num_col, td, fd, ht, acc_cell, acc_n, rec_key, rec_full, rotated;
+};
+
+/* Atomic/plain accumulation, supplied by the caller (GPU atomics or host adds). */
+struct DsmcOpsHost {
+    static inline void add_f64(double *p, double v) { *p += v; }
+    static inline void add_u64(unsigned long long *p, unsigned long long v) { *p += v; }
+    static inline int32_t claim_i32(int32_t *p) { return (*p)++; }
+    static inline unsigned long long claim_u64(unsigned long long *p) { return (*p)++; }
+};
+
+/* ------------------------------------------------------------------ RNG */
+DSMC_HD uint32_t lane_word(DsmcLane &L, const DsmcEnv &E)
+{
+    uint64_t b = L.k >> 2;
+    if (b != L.blk) {
+        dsmc_u32x4 r = dsmc_rng_block(E.seed, (uint64_t)L.pid, E.stream, b);
+        L.rb0 = r.w[0]; L.rb1 = r.w[1]; L.rb2 = r.w[2]; L.rb3 = r.w[3];
+        L.blk = b;
+    }
+    uint32_t j = (uint32_t)(L.k & 3);
+    L.k++;
+    return j == 0 ? L.rb0 : j == 1 ? L.rb1 : j == 2 ? L.rb2 : L.rb3;
+}
+DSMC_HD int lane_draw(DsmcLane &L, const DsmcEnv &E) { return (int)(lane_word(L, E) >> 1); }
+DSMC_HD double lane_u01(DsmcLane &L, const DsmcEnv &E)
+{
+    uint32_t hi = lane_word(L, E);
+    uint32_t lo = lane_word(L, E);
+    return dsmc_u01_from_words(hi, lo);
+}
+DSMC_HD double lane_urand(DsmcLane &L, const DsmcEnv &E)   /* (double)rand_r / RAND_MAX */
+{
+    return (double)lane_draw(L, E) / 2147483647;
+}
+
+/* ------------------------------------------------------------------ cells */
+/* grid.c:28-39 */
+DSMC_HD void sph_index(double thi, double fi, int ftd, int ffd, int *td, int *fd)
+{
+    *td = (int)ceil((thi + M_PI / 2.0) / M_PI * ftd) - 1;
+    *fd = (int)ceil(fi / (2.0 * M_PI) * ffd) - 1;
+    if (*td < 0) *td = 0;
+    if (*td >= ftd) *td = ftd - 1;
+    if (*fd < 0) *fd = 0;
+    if (*fd >= ffd) *fd = ffd - 1;
+}
+
+/* ballistic.c:29-33 with the device-math elementary functions */
+DSMC_HD void pos_angles(double x, double y, double z, double Ri, double *thi, double *fi)
+{
+    *thi = dm_asin(z / Ri);
+    *fi = dm_atan2(x, y);
+    if (*fi < 0) *fi += 2.0 * M_PI;
+}
+
+DSMC_HD int fd_exact(double x, double y, int ffd)
+{
+    double fi = dm_atan2(x, y);
+    if (fi < 0) fi += 2.0 * M_PI;
+    int fd = (int)ceil(fi / (2.0 * M_PI) * ffd) - 1;
+    if (fd < 0) fd = 0;
+    if (fd >= ffd) fd = ffd - 1;
+    return fd;
+}
+
+/* Longitude sector of (x, y): unchanged if safely inside sector `fd`. */
+DSMC_HD int fd_update(double x, double y, int fd, int ffd, const double *cs)
+{
+    double r = fabs(x) + fabs(y);
+    double eps = DSMC_FD_MARGIN * r;
+    double dlo = x * cs[2 * fd] - y * cs[2 * fd + 1];
+    double dhi = x * cs[2 * fd + 2] - y * cs[2 * fd + 3];
+    if (dlo > eps && dhi < -eps) return fd;
+    return fd_exact(x, y, ffd);
+}
+
+/* Index k with tab[k] <= v < tab[k+1], starting from a nearby guess. */
+DSMC_HD int table_walk(const double *tab, int k, int kmax, double v)
+{
+    while (k > 0 && v < tab[k]) k--;
+    while (k < kmax && v >= tab[k + 1]) k++;
+    return k;
+}
+
+/* ------------------------------------------------------------------ deposit */
+template <class Ops>
+DSMC_HD void lane_flush(DsmcLane &L, const DsmcEnv &E)
+{
+    if (L.acc_cell >= 0) {
+        Ops::add_f64(&E.density[L.acc_cell], L.acc_w);
+        if (E.vmom) {
+            Ops::add_f64(&E.vmom[(size_t)L.acc_cell * 3 + 0], L.acx);
+            Ops::add_f64(&E.vmom[(size_t)L.acc_cell * 3 + 1], L.acy);
+            Ops::add_f64(&E.vmom[(size_t)L.acc_cell * 3 + 2], L.acz);
+        }
+        if (E.vcount) Ops::add_u64(&E.vcount[L.acc_cell], (unsigned long long)L.acc_n);
+    }
+    L.acc_cell = -1;
+}
+
+template <class Ops>
+DSMC_HD void lane_deposit(DsmcLane &L, const DsmcEnv &E, int cell)
+{
+    if (cell != L.acc_cell) {
+        lane_flush<Ops>(L, E);
+        L.acc_cell = cell;
+        L.acc_w = 0.0; L.acx = 0.0; L.acy = 0.0; L.acz = 0.0; L.acc_n = 0;
+    }
+    L.acc_w += L.W;
+    L.acx += L.vx; L.acy += L.vy; L.acz += L.vz;
+    L.acc_n++;
+}
+
+/* ballistic.c:215-232: capped velocity records, first come first served per
+ * (td, ht) cell (cap ignores fd) and globally. */
+template <class Ops>
+DSMC_HD void lane_record(DsmcLane &L, const DsmcEnv &E)
+{
+    int key = L.td * E.hd + L.ht;
+    if (key != L.rec_key) { L.rec_key = key; L.rec_full = 0; }
+    if (L.rec_full) return;
+    int32_t slot = Ops::claim_i32(&E.count_nn[key]);
+    if (slot >= E.rec_cap_cell) { L.rec_full = 1; return; }
+    unsigned long long g = Ops::claim_u64(E.rec_total);
+    if ((int64_t)g >= E.rec_cap_total) { L.rec_full = 1; return; }
+    E.vrec[g * 3 + 0] = L.vx;
+    E.vrec[g * 3 + 1] = L.vy;
+    E.vrec[g * 3 + 2] = L.vz;
+    E.vrec_td[g] = L.td;
+    E.vrec_fd[g] = L.fd;
+    E.vrec_ht[g] = L.ht;
+}
+
+/* ------------------------------------------------------------------ end of a particle */
+struct DsmcCounts {
+    unsigned long long esc, frozen, photo, depleted, capped, coll, steps;
+};
+
+DSMC_HD void lane_finish(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
+{
+    int64_t n = L.pid;
+    if (E.ttime) {
+        int64_t N = E.nparticles;
+        if (L.fate != 1 /* escaped: rows 1+ stay 0 */) {
+            E.ttime[1 * N + n] = L.thi;
+            E.ttime[2 * N + n] = L.time;
+            E.ttime[3 * N + n] = L.W;
+            E.ttime[4 * N + n] = L.flight_time;
+            if (E.ttime_rows >= 9) {
+                E.ttime[7 * N + n] = L.rest_time;
+                E.ttime[8 * N + n] = (double)L.num_col;
+            }
+        }
+    }
+    if (E.fate_out) E.fate_out[n] = L.fate;
+    if (E.steps_out) E.steps_out[n] = L.steps;
+    if (E.hops_out) E.hops_out[n] = L.hops;
+    if (E.end_state) {
+        double *e = E.end_state + (size_t)n * 6;
+        if (L.steps == 0) { e[0] = e[1] = e[2] = e[3] = e[4] = e[5] = 0.0; }
+        else if (L.rotated) { e[0] = L.lsx; e[1] = L.lsy; e[2] = L.lsz; e[3] = L.vx; e[4] = L.vy; e[5] = L.vz; }
+        else { e[0] = L.sx; e[1] = L.sy; e[2] = L.sz; e[3] = L.vx; e[4] = L.vy; e[5] = L.vz; }
+    }
+    cnt.steps += (unsigned long long)L.steps;
+    cnt.coll += (unsigned long long)L.num_col;
+    switch (L.fate) {
+    case 1: cnt.esc++; break;
+    case 2: cnt.frozen++; break;
+    case 3: cnt.photo++; break;
+    case 4: cnt.depleted++; break;
+    case 5: cnt.capped++; break;
+    default: break;
+    }
+    L.phase = LANE_DONE;
+}
+
+/* ------------------------------------------------------------------ hop start */
+DSMC_HD void lane_begin_flight(DsmcLane &L, const DsmcEnv &E)
+{
+    double Ri = sqrt(L.sx * L.sx + L.sy * L.sy + L.sz * L.sz);
+    L.ox = E.GM * L.sx / (Ri * Ri * Ri);
+    L.oy = E.GM * L.sy / (Ri * Ri * Ri);
+    L.oz = E.GM * L.sz / (Ri * Ri * Ri);
+    L.Ri = Ri;
+    L.ii = 0;
+    /* current cell guesses for the incremental walks */
+    double q = L.sz / Ri;
+    L.td = table_walk(E.tq, E.ftd / 2, E.ftd - 1, q);
+    L.ht = table_walk(E.hr, 0, E.hd, Ri);
+    L.fd = fd_exact(L.sx, L.sy, E.ffd);
+    L.phase = LANE_FLIGHT;
+}
+
+/* ------------------------------------------------------------------ compat launch */
+/* Body of `while (W >= 0.01)` up to the flight loop: ballistic.c:124-186 and
+ * advanced.c:271-338. */
+template <class Ops, int ADV>
+DSMC_HD void lane_launch_compat(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
+{
+    const int ftd = E.ftd, ffd = E.ffd;
+    if (!(L.W >= E.w_min)) { L.fate = 4; lane_finish(L, E, cnt); return; }
+    double KF = 2.0 * M_PI * E.R * fabs(dm_cos(L.thi)) / E.period;
+    int td, fd;
+    sph_index(L.thi, L.fi, ftd, ffd, &td, &fd);
+    double local_T = E.T[td * ffd + fd];
+    if (local_T < E.t_freeze) {
+        int found = 0;
+        if (!ADV) {                                  /* ballistic.c:132-153 */
+            for (int add = 0; add < ffd; add++) {
+                L.time += E.dt_rot;
+                int cfd = (fd + add + 1) % ffd;
+                if (E.T[td * ffd + cfd] >= E.t_freeze) {
+                    fd = cfd;
+                    L.fi = (fd + 0.5) / ffd * 2.0 * M_PI;
+                    found = 1;
+                    break;
+                }
+            }
+        } else {                                     /* advanced.c:278-309 */
+            Ops::add_f64(&E.horizon[td * ffd + fd], L.W);
+            for (int add = 1; add <= ffd; add++) {
+                L.time += E.dt_rot;
+                L.rest_time += E.dt_rot;
+                if (E.T[td * ffd + (add % ffd)] >= E.t_freeze) {
+                    double rot_fi = (double)(add - fd) / ffd * 2.0 * M_PI;
+                    double sn, cs;
+                    dm_sincos(rot_fi, &sn, &cs);
+                    if (!L.rotated) { L.lsx = L.sx; L.lsy = L.sy; L.lsz = L.sz; L.rotated = 1; }
+                    double nx = L.sx * cs - L.sy * sn;
+                    double ny = L.sx * sn + L.sy * cs;
+                    L.sx = nx; L.sy = ny;
+                    fd = add % ffd;
+                    L.fi = (fd + 0.5) / ffd * 2.0 * M_PI;
+                    found = 1;
+                    break;
+                }
+            }
+        }
+        if (!found) {
+            Ops::add_f64(&E.horizon[td * ffd + fd], L.W);
+            L.fate = 2;
+            lane_finish(L, E, cnt);
+            return;
+        }
+        local_T = E.T[td * ffd + fd];
+    }
+    /* velocity.c:49-68 */
+    int idx;
+    if (local_T >= 220.0) idx = 20;
+    else idx = (int)ceil(local_T / 10.0) - 2;
+    if (idx < 0) idx = 0;
+    if (idx >= DSMC_VT_NT) idx = DSMC_VT_NT - 1;
+    int r = lane_draw(L, E) % E.vt_count[idx];
+    double vr = E.vt_speeds[idx * DSMC_VT_NV + r];
+    double v_old_sq = L.vx * L.vx + L.vy * L.vy + L.vz * L.vz;
+    double vout = sqrt(E.alpha * vr * vr + (1.0 - E.alpha) * v_old_sq);   /* velocity.c:70-73 */
+    double flyd = lane_urand(L, E) * M_PI + L.thi - M_PI / 2.0;
+    double xi = lane_urand(L, E) * M_PI + L.fi - M_PI / 2.0;
+    double sfl, cfl, sxi, cxi, sr, cr;
+    dm_sincos(flyd, &sfl, &cfl);
+    dm_sincos(xi, &sxi, &cxi);
+    dm_sincos(L.fi + M_PI / 2.0, &sr, &cr);
+    double nvx = cfl * sxi * vout;
+    double nvy = cfl * cxi * vout;
+    double nvz = sfl * vout;
+    nvx += KF * sr;
+    nvy += KF * cr;
+    L.hops++;
+    if (sqrt(nvx * nvx + nvy * nvy + nvz * nvz) >= E.ve) {   /* ballistic.c:173 */
+        L.fate = 1;
+        lane_finish(L, E, cnt);
+        return;
+    }
+    L.vx = nvx; L.vy = nvy; L.vz = nvz;
+    L.rotated = 0;
+    lane_begin_flight(L, E);
+}
+
+/* ------------------------------------------------------------------ physical emission */
+DSMC_HD void lane_emit(DsmcLane &L, const DsmcEnv &E, double T)
+{
+    double sig = sqrt(E.k_B * T / E.mass);
+    double u1 = lane_u01(L, E);
+    double u2 = lane_u01(L, E);
+    double w = -dm_log(u1 * u2);
+    double speed = sig * sqrt(2.0 * w);
+    double u3 = lane_u01(L, E);
+    double cos_t = sqrt(u3), sin_t = sqrt(1.0 - u3);
+    double phi = 2.0 * M_PI * lane_u01(L, E);
+    double sph, cph;
+    dm_sincos(phi, &sph, &cph);
+    double Ri = sqrt(L.sx * L.sx + L.sy * L.sy + L.sz * L.sz);
+    double nx = L.sx / Ri, ny = L.sy / Ri, nz = L.sz / Ri;
+    double hxy = sqrt(nx * nx + ny * ny);
+    double e1x, e1y, e1z;
+    if (hxy > 1e-12) { e1x = -ny / hxy; e1y = nx / hxy; e1z = 0.0; }
+    else { e1x = 1.0; e1y = 0.0; e1z = 0.0; }
+    double e2x = ny * e1z - nz * e1y;
+    double e2y = nz * e1x - nx * e1z;
+    double e2z = nx * e1y - ny * e1x;
+    double a = sin_t * cph, b = sin_t * sph;
+    L.vx = speed * (a * e1x + b * e2x + cos_t * nx) + E.omega * L.sy;
+    L.vy = speed * (a * e1y + b * e2y + cos_t * ny) - E.omega * L.sx;
+    L.vz = speed * (a * e1z + b * e2z + cos_t * nz);
+}
+
+DSMC_HD double face_margin(const DsmcEnv &E, int f, double dx, double dy, double dz)
+{
+    const double *n = E.enorm + (size_t)f * 9;
+    double m0 = dx * n[0] + dy * n[1] + dz * n[2];
+    double m1 = dx * n[3] + dy * n[4] + dz * n[5];
+    double m2 = dx * n[6] + dy * n[7] + dz * n[8];
+    double m = m0 < m1 ? m0 : m1;
+    return m < m2 ? m : m2;
+}
+
+/* Finest icosphere face containing direction d (largest minimum edge margin at
+ * every level of the subdivision hierarchy, lowest index on ties). */
+DSMC_HD int locate_face(const DsmcEnv &E, double dx, double dy, double dz)
+{
+    int best = 0;
+    double bm = face_margin(E, 0, dx, dy, dz);
+    for (int f = 1; f < 20; f++) {
+        double m = face_margin(E, f, dx, dy, dz);
+        if (m > bm) { bm = m; best = f; }
+    }
+    for (int l = 1; l <= E.levels; l++) {
+        int base = 4 * best, pick = base;
+        double pm = face_margin(E, E.level_off[l] + base, dx, dy, dz);
+        for (int j = 1; j < 4; j++) {
+            double m = face_margin(E, E.level_off[l] + base + j, dx, dy, dz);
+            if (m > pm) { pm = m; pick = base + j; }
+        }
+        best = pick;
+    }
+    return best;
+}
+
+/* ------------------------------------------------------------------ particle start */
+template <class Ops, int MODE, int ADV>
+DSMC_HD void lane_start(DsmcLane &L, const DsmcEnv &E, int64_t pid, DsmcCounts &cnt)
+{
+    L.pid = pid;
+    L.k = 0; L.blk = ~0ull;
+    L.W = 1.0; L.time = 0.0; L.flight_time = 0.0; L.rest_time = 0.0;
+    L.steps = 0; L.hops = 0; L.num_col = 0; L.n_day = 0;
+    L.fate = 0; L.rotated = 0;
+    L.acc_cell = -1; L.rec_key = -1; L.rec_full = 0;
+    L.vx = 0.0; L.vy = 0.0; L.vz = 0.0;
+    L.real_col_p = 0.0; L.col_p = 0.0; L.col_lo = 0.0;
+    if (MODE == 0) {
+        if (ADV) {
+            L.col_p = lane_urand(L, E);               /* advanced.c:246 */
+            /* 1 - exp(-x) > col_p needs x > -log(1 - col_p); below this bound
+             * (with a wide relative margin) the exact test cannot pass */
+            double omc = 1.0 - L.col_p + 1e-15;
+            L.col_lo = omc >= 1.0 ? 0.0 : -dm_log(omc) * (1.0 - 1e-9);
+        }
+        L.thi = lane_urand(L, E) * M_PI - M_PI / 2.0;
+        L.fi = lane_urand(L, E) * 2.0 * M_PI;
+        double st, ct, sf, cf;
+        dm_sincos(L.thi, &st, &ct);
+        dm_sincos(L.fi, &sf, &cf);
+        L.sx = E.R * ct * sf;
+        L.sy = E.R * ct * cf;
+        L.sz = E.R * st;
+        if (E.ttime) {
+            E.ttime[0 * E.nparticles + pid] = L.thi;
+            if (E.ttime_rows >= 9) {
+                E.ttime[5 * E.nparticles + pid] = L.fi;
+                int td, fd;
+                sph_index(L.thi, L.fi, E.ftd, E.ffd, &td, &fd);
+                E.ttime[6 * E.nparticles + pid] = E.T[td * E.ffd + fd];
+            }
+        }
+        lane_launch_compat<Ops, ADV>(L, E, cnt);
+        return;
+    }
+    /* physical: face by inverse CDF of the launch weights */
+    const int nf = E.nfine;
+    double x = lane_u01(L, E) * E.cdf[nf - 1];
+    int lo = 0, hi = nf - 1;
+    while (lo < hi) { int mid = (lo + hi) / 2; if (E.cdf[mid] > x) hi = mid; else lo = mid + 1; }
+    int face = lo;
+    const int32_t *fv = E.faces + 3 * (size_t)(E.level_off[E.levels] + face);
+    const double *a = E.vert + 3 * (size_t)fv[0];
+    const double *b = E.vert + 3 * (size_t)fv[1];
+    const double *e = E.vert + 3 * (size_t)fv[2];
+    double u1 = lane_u01(L, E);
+    double u2 = lane_u01(L, E);
+    double r1 = sqrt(u1);
+    double wa = 1.0 - r1, wb = r1 * (1.0 - u2), wc = r1 * u2;
+    double px = wa * a[0] + wb * b[0] + wc * e[0];
+    double py = wa * a[1] + wb * b[1] + wc * e[1];
+    double pz = wa * a[2] + wb * b[2] + wc * e[2];
+    double sc = E.R / sqrt(px * px + py * py + pz * pz);
+    L.sx = px * sc; L.sy = py * sc; L.sz = pz * sc;
+    double Ex = -dm_log(lane_u01(L, E));
+    double nph = ceil(Ex * E.lifetime / E.h);
+    L.n_photo = nph > 9.0e18 ? (int64_t)9000000000000000000LL : (int64_t)nph;
+    lane_emit(L, E, E.face_T[face]);
+    L.hops = 1;
+    lane_begin_flight(L, E);
+}
+
+/* ------------------------------------------------------------------ service */
+/* After a flight ends on the surface (or the per-hop step limit). */
+template <class Ops, int MODE, int ADV>
+DSMC_HD void lane_service(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
+{
+    lane_flush<Ops>(L, E);
+    if (MODE == 0) {                         /* ballistic.c:239-256 / advanced.c:405-420 */
+        double Ri = sqrt(L.sx * L.sx + L.sy * L.sy + L.sz * L.sz);
+        pos_angles(L.sx, L.sy, L.sz, Ri, &L.thi, &L.fi);
+        int td, fd;
+        sph_index(L.thi, L.fi, E.ftd, E.ffd, &td, &fd);
+        if (td <= 1 || td >= E.ftd - 2) {
+            Ops::add_f64(&E.horizon[td * E.ffd + fd], 0.05 * L.W);
+            L.W *= 0.95;
+        }
+        lane_launch_compat<Ops, ADV>(L, E, cnt);
+        return;
+    }
+    int64_t cap = E.max_steps > 0 ? E.max_steps : INT64_MAX;
+    if (L.steps >= cap) { L.fate = 5; lane_finish(L, E, cnt); return; }
+    int f = locate_face(E, L.sx, L.sy, L.sz);
+    double Tf = E.face_T[f];
+    if (Tf < E.t_freeze) {
+        double Ri = sqrt(L.sx * L.sx + L.sy * L.sy + L.sz * L.sz), thi, fi;
+        pos_angles(L.sx, L.sy, L.sz, Ri, &thi, &fi);
+        int td, fd;
+        sph_index(thi, fi, E.ftd, E.ffd, &td, &fd);
+        Ops::add_f64(&E.horizon[td * E.ffd + fd], L.W);
+        L.fate = 2;
+        lane_finish(L, E, cnt);
+        return;
+    }
+    double sc = E.R / sqrt(L.sx * L.sx + L.sy * L.sy + L.sz * L.sz);
+    L.sx *= sc; L.sy *= sc; L.sz *= sc;
+    lane_emit(L, E, Tf);
+    L.hops++;
+    lane_begin_flight(L, E);
+}
+
+/* ------------------------------------------------------------------ collision (advanced) */
+DSMC_HD double poly_eval(const double *c, int deg, double x)
+{
+    double v = 0.0;
+    for (int i = 0; i <= deg; i++) v = v * x + c[i];
+    return v;
+}
+
+/* collision.c:43-101 */
+DSMC_HD double sample_speed(const double *c, int deg, double min_v, double max_v, double u, int nn)
+{
+    double val_at_max = poly_eval(c, deg, max_v);
+    double ranking = u * ((double)nn - val_at_max) + val_at_max;
+    double v0 = min_v, v1 = max_v;
+    for (int iter = 0; iter < 100; iter++) {
+        double vm = (v0 + v1) * 0.5;
+        double fm = poly_eval(c, deg, vm) - ranking;
+        double f1 = poly_eval(c, deg, v1) - ranking;
+        if (fm * f1 < 0) v0 = vm;
+        else v1 = vm;
+        if (fabs(v1 - v0) < 0.01) break;
+    }
+    return (v0 + v1) * 0.5;
+}
+
+/* ------------------------------------------------------------------ one flight step */
+/* integrator.c:4-55 followed by ballistic.c:195-236 / advanced.c:347-402, or the
+ * physical-mode step.  Returns with L.phase = FLIGHT, LANDED or DONE. */
+template <class Ops, int MODE, int ADV>
+DSMC_HD void lane_step(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
+{
+    const double h = E.h;
+    if (MODE == 0) { L.flight_time += h; L.time += h; }
+    L.steps++;
+    double Ri = L.Ri;
+    double Ri3 = Ri * Ri * Ri;
+    double g = E.GM / Ri3;
+    double ax = g * L.sx, ay = g * L.sy, az = g * L.sz;
+    double nx = L.sx + L.vx * h - ax * E.c1 - L.ox * E.c2;
+    double ny = L.sy + L.vy * h - ay * E.c1 - L.oy * E.c2;
+    double nz = L.sz + L.vz * h - az * E.c1 - L.oz * E.c2;
+    double ds = 0.0;
+    if (ADV) {
+        double dx = nx - L.sx, dy = ny - L.sy, dz = nz - L.sz;
+        ds = sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    L.vx -= ax * h;
+    L.vy -= ay * h;
+    L.vz -= az * h;
+    L.ox = ax; L.oy = ay; L.oz = az;
+    L.sx = nx; L.sy = ny; L.sz = nz;
+    double Rn = sqrt(nx * nx + ny * ny + nz * nz);
+    L.Ri = Rn;
+    if (Rn <= E.R) { L.phase = LANE_LANDED; return; }
+    if (Rn >= E.max_height) { lane_flush<Ops>(L, E); L.fate = 1; lane_finish(L, E, cnt); return; }
+
+    if (MODE != 0) {
+        if (nx * E.sun_x + ny * E.sun_y > 0.0) {
+            L.n_day++;
+            if (L.n_day >= L.n_photo) { lane_flush<Ops>(L, E); L.fate = 3; lane_finish(L, E, cnt); return; }
+        }
+        L.td = table_walk(E.tq, L.td, E.ftd - 1, nz / Rn);
+        L.fd = fd_update(nx, ny, L.fd, E.ffd, E.cs);
+        L.ht = table_walk(E.hr, L.ht, E.hd, Rn);
+        if (L.ht < E.hd) lane_deposit<Ops>(L, E, (L.ht * E.ftd + L.td) * E.ffd + L.fd);
+        int64_t cap = E.max_steps > 0 ? E.max_steps : INT64_MAX;
+        if (L.steps >= cap) { lane_flush<Ops>(L, E); L.fate = 5; lane_finish(L, E, cnt); }
+        return;
+    }
+
+    L.td = table_walk(E.tq, L.td, E.ftd - 1, nz / Rn);
+    L.fd = fd_update(nx, ny, L.fd, E.ffd, E.cs);
+    if (L.fd < E.ffd / 2) L.W *= E.decay;
+    int64_t ii = L.ii++;
+    if (ii + 1 >= E.max_hop_steps) L.phase = LANE_LANDED;   /* loop bound reached */
+    if (ii == 0) return;
+    L.ht = table_walk(E.hr, L.ht, E.hd, Rn);
+    if (L.ht >= E.hd) return;
+    if (ADV) {
+        if (E.copy_atm && L.ht < E.hd / 3) {           /* advanced.c:369-398 */
+            int64_t atm_idx = (int64_t)L.ht * E.ftd * E.ffd + L.td * E.ffd + L.fd;
+            double atm = atm_idx < E.atm_total ? E.copy_atm[atm_idx] : 0.0;
+            L.real_col_p += ds * atm * E.sigma;
+            if (L.real_col_p >= L.col_lo) {
+                int pidx = L.td * E.ffd * E.hd + L.fd * E.hd + L.ht;
+                if (L.col_p < 1.0 - dm_exp(-L.real_col_p) && E.pvalid[pidx]) {
+                    L.real_col_p = 0.0;
+                    L.num_col++;
+                    double u = lane_urand(L, E);
+                    double vv2 = E.pvalid[pidx]
+                        ? sample_speed(E.coeffs + (size_t)pidx * (E.fit_deg + 1), E.fit_deg,
+                                       E.pmin[pidx], E.pmax[pidx], u, E.count_num[pidx])
+                        : 0.0;
+                    double thi2 = M_PI * (double)lane_draw(L, E) / 2147483647 - M_PI / 2.0;
+                    double fi2 = 2.0 * M_PI * (double)lane_draw(L, E) / 2147483647;
+                    double st2, ct2, sf2, cf2;
+                    dm_sincos(thi2, &st2, &ct2);
+                    dm_sincos(fi2, &sf2, &cf2);
+                    double v2x = vv2 * ct2 * cf2, v2y = vv2 * ct2 * sf2, v2z = vv2 * st2;
+                    double dvx = L.vx - v2x, dvy = L.vy - v2y, dvz = L.vz - v2z;   /* collision.c:35-40 */
+                    double cvx = L.vx + v2x, cvy = L.vy + v2y, cvz = L.vz + v2z;
+                    L.vx = cvx + 0.5 * dvx;
+                    L.vy = cvy + 0.5 * dvy;
+                    L.vz = cvz + 0.5 * dvz;
+                    return;
+                }
+            }
+        }
+    } else if (E.rec_cap_cell > 0 && E.vrec) {
+        lane_record<Ops>(L, E);
+    }
+    lane_deposit<Ops>(L, E, (L.ht * E.ftd + L.td) * E.ffd + L.fd);
+}
+
+#endif /* DSMC_LANE_H */

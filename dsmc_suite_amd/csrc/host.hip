@@ -1,0 +1,521 @@
+/*
+ * host.hip - the C ABI of libdsmc_hip.so (include/dsmc_hip.h).
+ *
+ * Host side of a run: derive the run's scalars and exact index tables on the CPU
+ * (dsmc_tables.h), upload them with the surface, launch the persistent particle
+ * kernel on one HIP stream, copy the results back.  The drop-in run_ballistic /
+ * run_advanced wrap this with the reference's parameters and .dat files.
+ */
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <time.h>
+#include <vector>
+#include <mutex>
+
+#include "../../include/dsmc_hip.h"
+#include "dsmc_tables.h"
+#include "dsmc_kernels.h"
+#include "dsmc_host.h"
+
+#define HIP_OK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            fprintf(stderr, "dsmc_hip: %s failed: %s (%s:%d)\n", #expr,                  \
+                    hipGetErrorString(e_), __FILE__, __LINE__);                           \
+            return DSMC_ERR_DEVICE;                                                       \
+        }                                                                                 \
+    } while (0)
+
+namespace dsmc {
+
+/* ------------------------------------------------------------------ device buffers */
+DevBuf::~DevBuf() { release(); }
+void DevBuf::release()
+{
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+int DevBuf::alloc(size_t n)
+{
+    release();
+    if (n == 0) return DSMC_OK;
+    if (hipMalloc(&ptr, n) != hipSuccess) { ptr = nullptr; return DSMC_ERR_NOMEM; }
+    bytes = n;
+    return DSMC_OK;
+}
+int DevBuf::upload(const void *src, size_t n, hipStream_t s)
+{
+    int rc = alloc(n);
+    if (rc || n == 0) return rc;
+    HIP_OK(hipMemcpyAsync(ptr, src, n, hipMemcpyHostToDevice, s));
+    return DSMC_OK;
+}
+int DevBuf::zeros(size_t n, hipStream_t s)
+{
+    int rc = alloc(n);
+    if (rc || n == 0) return rc;
+    HIP_OK(hipMemsetAsync(ptr, 0, n, s));
+    return DSMC_OK;
+}
+
+int device_props(int *cus)
+{
+    static int cached = -1;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> g(mu);
+    if (cached < 0) {
+        int dev = 0;
+        HIP_OK(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        HIP_OK(hipGetDeviceProperties(&prop, dev));
+        if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            fprintf(stderr, "dsmc_hip: warning: device is %s, kernels are built for gfx950\n",
+                    prop.gcnArchName);
+        cached = prop.multiProcessorCount;
+    }
+    *cus = cached;
+    return DSMC_OK;
+}
+
+/* ------------------------------------------------------------------ run staging */
+/* Uploads everything a particle kernel reads and allocates what it writes. */
+int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r, int advanced,
+              RunDev *rd, hipStream_t st)
+{
+    int rc = build_env(p, s, advanced, &rd->E, &rd->T);
+    if (rc) return rc;
+    DsmcEnv &E = rd->E;
+    HostTables &T = rd->T;
+    const size_t cells = (size_t)p->hd * p->ftd * p->ffd;
+    const int64_t n = p->nparticles;
+#define UP(buf, vec) do { if ((rc = (buf).upload((vec).data(), (vec).size() * sizeof((vec)[0]), st))) return rc; } while (0)
+    UP(rd->tq, T.tq);
+    UP(rd->hr, T.hr);
+    UP(rd->cs, T.cs);
+    E.tq = (const double *)rd->tq.ptr;
+    E.hr = (const double *)rd->hr.ptr;
+    E.cs = (const double *)rd->cs.ptr;
+    if (p->mode == DSMC_MODE_COMPAT) {
+        UP(rd->vt_speeds, T.vt_speeds);
+        UP(rd->vt_count, T.vt_count);
+        if ((rc = rd->Tsurf.upload(s->T_latlon, sizeof(double) * p->ftd * p->ffd, st))) return rc;
+        E.vt_speeds = (const double *)rd->vt_speeds.ptr;
+        E.vt_count = (const int32_t *)rd->vt_count.ptr;
+        E.T = (const double *)rd->Tsurf.ptr;
+    } else {
+        UP(rd->cdf, T.cdf);
+        UP(rd->enorm, T.enorm);
+        if ((rc = rd->vert.upload(s->vert, sizeof(double) * 3 * s->nvert, st))) return rc;
+        if ((rc = rd->faces.upload(s->faces, sizeof(int32_t) * 3 * T.nface_all, st))) return rc;
+        if ((rc = rd->face_T.upload(s->face_T, sizeof(double) * T.nfine, st))) return rc;
+        E.cdf = (const double *)rd->cdf.ptr;
+        E.enorm = (const double *)rd->enorm.ptr;
+        E.vert = (const double *)rd->vert.ptr;
+        E.faces = (const int32_t *)rd->faces.ptr;
+        E.face_T = (const double *)rd->face_T.ptr;
+    }
+#undef UP
+    if ((rc = rd->density.zeros(cells * sizeof(double), st))) return rc;
+    if ((rc = rd->horizon.zeros(sizeof(double) * p->ftd * p->ffd, st))) return rc;
+    E.density = (double *)rd->density.ptr;
+    E.horizon = (double *)rd->horizon.ptr;
+    if (r->vmom) {
+        if ((rc = rd->vmom.zeros(cells * 3 * sizeof(double), st))) return rc;
+        E.vmom = (double *)rd->vmom.ptr;
+    }
+    if (r->vcount) {
+        if ((rc = rd->vcount.zeros(cells * sizeof(unsigned long long), st))) return rc;
+        E.vcount = (unsigned long long *)rd->vcount.ptr;
+    }
+    int rows = advanced ? 9 : 5;
+    E.ttime_rows = rows;
+    if (r->ttime && p->mode == DSMC_MODE_COMPAT) {
+        if ((rc = rd->ttime.zeros(sizeof(double) * rows * n, st))) return rc;
+        E.ttime = (double *)rd->ttime.ptr;
+    }
+    if (r->end_state) {
+        if ((rc = rd->end_state.zeros(sizeof(double) * 6 * n, st))) return rc;
+        E.end_state = (double *)rd->end_state.ptr;
+    }
+    if (r->steps) {
+        if ((rc = rd->steps.zeros(sizeof(int64_t) * n, st))) return rc;
+        E.steps_out = (int64_t *)rd->steps.ptr;
+    }
+    if (r->fate) {
+        if ((rc = rd->fate.zeros(sizeof(int32_t) * n, st))) return rc;
+        E.fate_out = (int32_t *)rd->fate.ptr;
+    }
+    if (r->hops) {
+        if ((rc = rd->hops.zeros(sizeof(int32_t) * n, st))) return rc;
+        E.hops_out = (int32_t *)rd->hops.ptr;
+    }
+    if (!advanced && p->mode == DSMC_MODE_COMPAT && p->record_cap_cell > 0 && r->vrec &&
+        r->vrec_td && r->vrec_fd && r->vrec_ht && p->record_cap_total > 0) {
+        size_t cap = (size_t)p->record_cap_total;
+        if ((rc = rd->vrec.zeros(cap * 3 * sizeof(double), st))) return rc;
+        if ((rc = rd->vrec_i.zeros(cap * 3 * sizeof(int32_t), st))) return rc;
+        if ((rc = rd->count_nn.zeros(sizeof(int32_t) * p->ftd * p->hd, st))) return rc;
+        E.vrec = (double *)rd->vrec.ptr;
+        E.vrec_td = (int32_t *)rd->vrec_i.ptr;
+        E.vrec_fd = E.vrec_td + cap;
+        E.vrec_ht = E.vrec_fd + cap;
+        E.count_nn = (int32_t *)rd->count_nn.ptr;
+    } else {
+        E.rec_cap_cell = 0;
+    }
+    if ((rc = rd->ctl.zeros(sizeof(unsigned long long) * 16, st))) return rc;
+    E.rec_total = (unsigned long long *)rd->ctl.ptr + 8;
+    return DSMC_OK;
+}
+
+int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st)
+{
+    int cus = 256;
+    int rc = device_props(&cus);
+    if (rc) return rc;
+    dsmc_launch_shape sh = shape ? *shape : default_shape(rd->E);
+    if (sh.block <= 0 || sh.block % 64 || sh.block > 1024 || sh.blocks_per_cu <= 0) return DSMC_ERR_ARG;
+    int64_t waves_needed = (rd->E.nparticles + 63) / 64;
+    int64_t grid = (int64_t)cus * sh.blocks_per_cu;
+    int64_t max_grid = (waves_needed * 64 + sh.block - 1) / sh.block;
+    if (grid > max_grid) grid = max_grid;
+    if (grid < 1) grid = 1;
+    unsigned long long *ctl = (unsigned long long *)rd->ctl.ptr;
+    HIP_OK(dsmc_launch_particles(&rd->E, ctl, ctl + 1, (int)grid, sh.block, sh.relaunch_batch, st));
+    return DSMC_OK;
+}
+
+dsmc_launch_shape default_shape(const DsmcEnv &E)
+{
+    dsmc_launch_shape s;
+    s.block = 256;
+    s.blocks_per_cu = 2;
+    s.relaunch_batch = E.mode == 0 ? 24 : 16;
+    return s;
+}
+
+int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st)
+{
+    const size_t cells = (size_t)p->hd * p->ftd * p->ffd;
+    const int64_t n = p->nparticles;
+    unsigned long long ctl[16];
+    HIP_OK(hipMemcpyAsync(ctl, rd->ctl.ptr, sizeof ctl, hipMemcpyDeviceToHost, st));
+    if (r->density) HIP_OK(hipMemcpyAsync(r->density, rd->density.ptr, cells * 8, hipMemcpyDeviceToHost, st));
+    if (r->horizon)
+        HIP_OK(hipMemcpyAsync(r->horizon, rd->horizon.ptr, sizeof(double) * p->ftd * p->ffd,
+                              hipMemcpyDeviceToHost, st));
+    if (r->vmom && rd->vmom.ptr) HIP_OK(hipMemcpyAsync(r->vmom, rd->vmom.ptr, cells * 24, hipMemcpyDeviceToHost, st));
+    if (r->vcount && rd->vcount.ptr)
+        HIP_OK(hipMemcpyAsync(r->vcount, rd->vcount.ptr, cells * 8, hipMemcpyDeviceToHost, st));
+    if (r->ttime) {
+        if (rd->ttime.ptr)
+            HIP_OK(hipMemcpyAsync(r->ttime, rd->ttime.ptr, rd->ttime.bytes, hipMemcpyDeviceToHost, st));
+    }
+    if (r->end_state) HIP_OK(hipMemcpyAsync(r->end_state, rd->end_state.ptr, 48 * n, hipMemcpyDeviceToHost, st));
+    if (r->steps) HIP_OK(hipMemcpyAsync(r->steps, rd->steps.ptr, 8 * n, hipMemcpyDeviceToHost, st));
+    if (r->fate) HIP_OK(hipMemcpyAsync(r->fate, rd->fate.ptr, 4 * n, hipMemcpyDeviceToHost, st));
+    if (r->hops) HIP_OK(hipMemcpyAsync(r->hops, rd->hops.ptr, 4 * n, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    r->vrec_n = 0;
+    if (rd->E.vrec) {
+        int64_t nrec = (int64_t)ctl[8];
+        if (nrec > p->record_cap_total) nrec = p->record_cap_total;
+        size_t cap = (size_t)p->record_cap_total;
+        if (nrec > 0) {
+            HIP_OK(hipMemcpy(r->vrec, rd->vrec.ptr, nrec * 24, hipMemcpyDeviceToHost));
+            const int32_t *base = (const int32_t *)rd->vrec_i.ptr;
+            HIP_OK(hipMemcpy(r->vrec_td, base, nrec * 4, hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(r->vrec_fd, base + cap, nrec * 4, hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(r->vrec_ht, base + 2 * cap, nrec * 4, hipMemcpyDeviceToHost));
+        }
+        r->vrec_n = nrec;
+    }
+    if (r->ttime && !rd->ttime.ptr) memset(r->ttime, 0, sizeof(double) * 5 * n);
+    r->n_escaped = (int64_t)ctl[1];
+    r->n_frozen = (int64_t)ctl[2];
+    r->n_photolysed = (int64_t)ctl[3];
+    r->n_depleted = (int64_t)ctl[4];
+    r->n_capped = (int64_t)ctl[5];
+    r->n_collisions = (int64_t)ctl[6];
+    r->total_steps = (int64_t)ctl[7];
+    return DSMC_OK;
+}
+
+}  // namespace dsmc
+
+using namespace dsmc;
+
+/* ------------------------------------------------------------------ presets */
+extern "C" int dsmc_params_preset(dsmc_params *p, int preset, int ftd, int ffd)
+{
+    if (!p || ftd <= 0 || ffd <= 0) return DSMC_ERR_ARG;
+    memset(p, 0, sizeof(*p));
+    p->ftd = ftd;
+    p->ffd = ffd;
+    p->max_hop_steps = 10000000;            /* ballistic.c:188 */
+    p->w_min = 0.01;                        /* ballistic.c:124 */
+    p->density_scale = 1.0;
+    p->sublimation_scale = 1.0;
+    p->fit_degree = 3;                      /* types.h:89 POLY_DEGREE */
+    p->max_per_cell = 50;                   /* advanced.c:169 */
+    p->min_fit_samples = 8;                 /* advanced.c:44 */
+    if (preset == DSMC_PRESET_COMPAT_BALLISTIC || preset == DSMC_PRESET_COMPAT_ADVANCED) {
+        /* include/constants.h:5-35, ballistic.c:49-67, advanced.c:131-152 */
+        p->mode = DSMC_MODE_COMPAT;
+        p->G = 6.67e-11;
+        p->M = 9.39e20;
+        p->radius = 476.2e3;
+        p->k_B = 1.38e-23;
+        p->mmass = 18e-3;
+        p->mole = 6.02e23;
+        p->lifetime = 1.0 / 1.02e-5 * (2.8 * 2.8);
+        p->period = 9.07417 * 3600.0;
+        p->alpha = 0.5;
+        p->v_esc = sqrt(2.0 * p->G * p->M / p->radius);
+        p->t_freeze = 100.0;
+        p->cross_section = 1e-19;
+        if (preset == DSMC_PRESET_COMPAT_BALLISTIC) {
+            p->stream = DSMC_STREAM_BALLISTIC;
+            p->dt = 5.0;
+            p->hd = 100;
+            p->exp_grid = 0;
+            p->unih = p->radius * 0.1;
+            p->alpha2 = 0.0;
+            p->max_height = p->unih * p->hd + p->radius;
+        } else {
+            p->stream = DSMC_STREAM_ADVANCED;
+            p->dt = 0.5;
+            p->hd = 50;
+            p->exp_grid = 1;
+            p->max_height = 3e6 * sqrt(2.0);
+            p->alpha2 = 0.04;
+            p->unih = (p->max_height - p->radius) / (pow(1.0 + p->alpha2, p->hd) - 1.0) * p->alpha2;
+        }
+        p->record_cap_cell = 200;                               /* ballistic.c:216 */
+        p->record_cap_total = (int64_t)100 * ftd * p->hd;       /* ballistic.c:77 */
+        return DSMC_OK;
+    }
+    if (preset == DSMC_PRESET_CERES_PHYSICAL || preset == DSMC_PRESET_CERES_PHYSICAL_COLL) {
+        p->mode = DSMC_MODE_PHYSICAL;
+        p->stream = DSMC_STREAM_BALLISTIC;
+        p->G = 1.0;
+        p->M = 6.26e10;                      /* GM of Ceres [m^3/s^2] */
+        p->radius = 470e3;
+        p->k_B = 1.380649e-23;
+        p->mmass = 2.99e-26;                 /* H2O molecule [kg] */
+        p->mole = 1.0;
+        p->lifetime = 6.4e5;
+        p->period = 9.07417 * 3600.0;
+        p->alpha = 1.0;
+        p->v_esc = sqrt(2.0 * p->G * p->M / p->radius);
+        p->dt = preset == DSMC_PRESET_CERES_PHYSICAL ? 10.0 : 5.0;
+        p->max_hop_steps = 0;
+        p->max_steps = 20000;
+        p->t_freeze = 110.0;
+        p->w_min = 0.0;
+        p->subsolar_lon = 0.0;
+        p->hd = 50;
+        p->exp_grid = 1;
+        p->alpha2 = 0.04;
+        p->max_height = 20.0 * p->radius;
+        p->unih = (p->max_height - p->radius) / (pow(1.0 + p->alpha2, p->hd) - 1.0) * p->alpha2;
+        p->record_cap_cell = 0;
+        p->record_cap_total = 0;
+        p->cross_section = M_PI * 4.6e-10 * 4.6e-10;
+        p->fit_degree = 6;
+        if (preset == DSMC_PRESET_CERES_PHYSICAL_COLL) p->sublimation_scale = 10.0;
+        return DSMC_OK;
+    }
+    return DSMC_ERR_ARG;
+}
+
+/* ------------------------------------------------------------------ single runs */
+extern "C" int dsmc_run_ballistic(const dsmc_params *p, const dsmc_surface *s, dsmc_result *r)
+{
+    if (!p || !s || !r || p->nparticles < 0) return DSMC_ERR_ARG;
+    int dev_rc = ensure_device();
+    if (dev_rc) return dev_rc;
+    hipStream_t st;
+    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    RunDev rd;
+    int rc = stage_run(p, s, r, 0, &rd, st);
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    if (!rc) {
+        HIP_OK(hipEventRecord(e0, st));
+        rc = launch_run(&rd, nullptr, st);
+        HIP_OK(hipEventRecord(e1, st));
+    }
+    if (!rc) rc = collect_run(&rd, p, r, st);
+    if (!rc) {
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        r->elapsed_ms = ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+/* ------------------------------------------------------------------ staged runs */
+struct dsmc_run {
+    dsmc_params p;
+    RunDev rd;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int launched = 0;
+};
+
+extern "C" dsmc_run *dsmc_run_create(const dsmc_params *p, const dsmc_surface *s,
+                                     const dsmc_result *outputs, int *status)
+{
+    int rc = (!p || !s || !outputs || p->nparticles < 0) ? DSMC_ERR_ARG : ensure_device();
+    if (rc) { if (status) *status = rc; return nullptr; }
+    dsmc_run *run = new dsmc_run();
+    run->p = *p;
+    if (hipStreamCreateWithFlags(&run->st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&run->e0) != hipSuccess || hipEventCreate(&run->e1) != hipSuccess) {
+        rc = DSMC_ERR_DEVICE;
+    } else {
+        rc = stage_run(p, s, outputs, 0, &run->rd, run->st);
+        if (!rc && hipStreamSynchronize(run->st) != hipSuccess) rc = DSMC_ERR_DEVICE;
+    }
+    if (rc) { dsmc_run_destroy(run); run = nullptr; }
+    if (status) *status = rc;
+    return run;
+}
+
+/* Clears every output buffer of a staged run on its stream. */
+static int clear_outputs(RunDev &rd, hipStream_t st)
+{
+    DevBuf *bufs[] = {&rd.density, &rd.horizon, &rd.vmom, &rd.vcount, &rd.ttime, &rd.end_state,
+                      &rd.steps, &rd.fate, &rd.hops, &rd.count_nn, &rd.ctl};
+    for (DevBuf *b : bufs)
+        if (b->ptr) HIP_OK(hipMemsetAsync(b->ptr, 0, b->bytes, st));
+    return DSMC_OK;
+}
+
+extern "C" int dsmc_run_launch(dsmc_run *run, const dsmc_launch_shape *shape)
+{
+    if (!run) return DSMC_ERR_ARG;
+    int rc = clear_outputs(run->rd, run->st);
+    if (rc) return rc;
+    HIP_OK(hipEventRecord(run->e0, run->st));
+    rc = launch_run(&run->rd, shape, run->st);
+    if (rc) return rc;
+    HIP_OK(hipEventRecord(run->e1, run->st));
+    run->launched = 1;
+    return DSMC_OK;
+}
+
+extern "C" int dsmc_run_sync(dsmc_run *run)
+{
+    if (!run) return DSMC_ERR_ARG;
+    HIP_OK(hipStreamSynchronize(run->st));
+    return DSMC_OK;
+}
+
+extern "C" double dsmc_run_kernel_ms(dsmc_run *run)
+{
+    if (!run || !run->launched) return -1.0;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, run->e0, run->e1) != hipSuccess) return -1.0;
+    return ms;
+}
+
+extern "C" int dsmc_run_collect(dsmc_run *run, dsmc_result *r)
+{
+    if (!run || !r) return DSMC_ERR_ARG;
+    int rc = collect_run(&run->rd, &run->p, r, run->st);
+    if (!rc) r->elapsed_ms = dsmc_run_kernel_ms(run);
+    return rc;
+}
+
+extern "C" void dsmc_run_destroy(dsmc_run *run)
+{
+    if (!run) return;
+    if (run->st) (void)hipStreamSynchronize(run->st);
+    if (run->e0) (void)hipEventDestroy(run->e0);
+    if (run->e1) (void)hipEventDestroy(run->e1);
+    run->rd.~RunDev();
+    new (&run->rd) RunDev();
+    if (run->st) (void)hipStreamDestroy(run->st);
+    delete run;
+}
+
+/* ------------------------------------------------------------------ diagnostics */
+extern "C" int dsmc_verlet_steps(double *state, int n, int nsteps, double GM, double R,
+                                 double max_height, double h, double *traj)
+{
+    if (!state || n <= 0 || nsteps < 0) return DSMC_ERR_ARG;
+    int rc = ensure_device();
+    if (rc) return rc;
+    DevBuf ds, dt;
+    if ((rc = ds.upload(state, sizeof(double) * 9 * n, 0))) return rc;
+    if (traj && (rc = dt.alloc(sizeof(double) * 7 * (size_t)n * nsteps))) return rc;
+    HIP_OK(dsmc_launch_verlet((double *)ds.ptr, n, nsteps, GM, R, max_height, h,
+                              traj ? (double *)dt.ptr : nullptr, 0));
+    HIP_OK(hipMemcpy(state, ds.ptr, sizeof(double) * 9 * n, hipMemcpyDeviceToHost));
+    if (traj) HIP_OK(hipMemcpy(traj, dt.ptr, dt.bytes, hipMemcpyDeviceToHost));
+    return DSMC_OK;
+}
+
+extern "C" int dsmc_elementary(int fn, const double *x, const double *y, double *out, int n)
+{
+    if (!x || !out || n <= 0) return DSMC_ERR_ARG;
+    int rc = ensure_device();
+    if (rc) return rc;
+    DevBuf dx, dy, dout;
+    if ((rc = dx.upload(x, 8 * (size_t)n, 0))) return rc;
+    if (y && (rc = dy.upload(y, 8 * (size_t)n, 0))) return rc;
+    if ((rc = dout.alloc(8 * (size_t)n))) return rc;
+    HIP_OK(dsmc_launch_math(fn, (const double *)dx.ptr, y ? (const double *)dy.ptr : nullptr,
+                            (double *)dout.ptr, n, 0));
+    HIP_OK(hipMemcpy(out, dout.ptr, 8 * (size_t)n, hipMemcpyDeviceToHost));
+    return DSMC_OK;
+}
+
+extern "C" const char *dsmc_version(void) { return "dsmc_hip 0.1 (gfx950)"; }
+
+extern "C" int dsmc_device_info(char *buf, size_t len)
+{
+    if (!buf || len == 0) return DSMC_ERR_ARG;
+    int rc = ensure_device();
+    if (rc) return rc;
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_OK(hipGetDeviceProperties(&prop, dev));
+    snprintf(buf, len, "%s, %d CUs, %.1f GB HBM, clock %d kHz", prop.gcnArchName,
+             prop.multiProcessorCount, prop.totalGlobalMem / 1e9, prop.clockRate);
+    return DSMC_OK;
+}
+
+namespace dsmc {
+int ensure_device()
+{
+    static int state = 0;   /* 0 unknown, 1 ok, <0 error */
+    static std::mutex mu;
+    std::lock_guard<std::mutex> g(mu);
+    if (state == 0) {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+            fprintf(stderr, "dsmc_hip: no HIP device available\n");
+            state = DSMC_ERR_DEVICE;
+        } else {
+            const char *d = getenv("DSMC_DEVICE");
+            int dev = d ? atoi(d) : 0;
+            if (dev < 0 || dev >= n || hipSetDevice(dev) != hipSuccess) state = DSMC_ERR_DEVICE;
+            else state = 1;
+        }
+    }
+    return state == 1 ? DSMC_OK : state;
+}
+}  // namespace dsmc

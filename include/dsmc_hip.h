@@ -193,6 +193,33 @@ int dsmc_run_ballistic(const dsmc_params *p, const dsmc_surface *s, dsmc_result 
 int dsmc_run_advanced(const dsmc_params *p, const dsmc_surface *s,
                       const dsmc_phase2_input *in, dsmc_result *r);
 
+/* Launch shape of the persistent particle kernel. */
+typedef struct dsmc_launch_shape {
+    int block;                /* threads per block (multiple of 64) */
+    int blocks_per_cu;        /* resident blocks per CU */
+    int relaunch_batch;       /* lanes that must land before a wave relaunches them */
+} dsmc_launch_shape;
+
+/* Staged runs: inputs uploaded once and kept resident in device memory, the run
+ * relaunched any number of times (outputs are cleared before each launch).
+ * `outputs` only says which outputs to allocate (non-NULL pointers). */
+typedef struct dsmc_run dsmc_run;
+dsmc_run *dsmc_run_create(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *outputs,
+                          int *status);
+int dsmc_run_launch(dsmc_run *run, const dsmc_launch_shape *shape);  /* async on the run's stream */
+int dsmc_run_sync(dsmc_run *run);
+double dsmc_run_kernel_ms(dsmc_run *run);        /* HIP-event time of the last launch */
+int dsmc_run_collect(dsmc_run *run, dsmc_result *r);
+void dsmc_run_destroy(dsmc_run *run);
+
+/* Diagnostics on the device: independent Verlet steps (integrator.c:4-55) of
+ * n states [pos, vel, oa] x 9, optional trajectory [n][nsteps][pos, vel, status];
+ * and the device elementary functions (0 sin, 1 cos, 2 asin, 3 atan2(x, y),
+ * 4 exp, 5 log, 6 sqrt, 7 x / y). */
+int dsmc_verlet_steps(double *state, int n, int nsteps, double GM, double R, double max_height,
+                      double h, double *traj);
+int dsmc_elementary(int fn, const double *x, const double *y, double *out, int n);
+
 /* Per-cell polynomial CDF fits (advanced.c:32-117 + polyfit.c:10-101) computed on
  * the device from velocity records: outputs [ftd*ffd*hd] cells in the
  * reference's [td][fd][ht] order, coeffs[cell*(degree+1)], highest power first. */
@@ -204,11 +231,6 @@ int dsmc_build_fits(const dsmc_params *p, const dsmc_phase2_input *in,
 /* Runs `nruns` independent runs in one persistent launch.  params[i],
  * surfaces[i] and results[i] describe run i.  `tune` selects the launch shape:
  * NULL = autotuned (cached per device and workload class). */
-typedef struct dsmc_launch_shape {
-    int block;                /* threads per block (multiple of 64) */
-    int blocks_per_cu;        /* resident blocks per CU */
-    int relaunch_batch;       /* lanes that must land before a wave relaunches them */
-} dsmc_launch_shape;
 int dsmc_run_batch(int nruns, const dsmc_params *params, const dsmc_surface *surfaces,
                    dsmc_result *results, const dsmc_launch_shape *tune);
 int dsmc_autotune(const dsmc_params *p, const dsmc_surface *s, dsmc_launch_shape *out);

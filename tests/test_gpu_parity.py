@@ -1,0 +1,187 @@
+"""HIP path (libdsmc_hip.so, gfx950) against the oracle, through the C ABI.
+
+The device kernels and the oracle's device-math build evaluate the same
+arithmetic in the same order, so per-particle results are compared bit for bit;
+accumulated fields differ only by atomic summation order (rtol 1e-12).
+"""
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from dsmc_suite_amd import capi
+from dsmc_suite_amd.capi import (FATE_ESCAPED, PRESET_CERES_PHYSICAL, Buffers, SurfaceArrays, ptr)
+from tests.helpers import GOLDEN, compat_params, golden, oracle, read_T, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return capi.load()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    return oracle(devmath=True)
+
+
+def icosphere(lib, levels, lon=0.0, mass=2.99e-26, kB=1.380649e-23):
+    nv, na, nf = C.c_int32(), C.c_int32(), C.c_int32()
+    assert lib.dsmc_icosphere_sizes(levels, C.byref(nv), C.byref(na), C.byref(nf)) == 0
+    vert = np.zeros(nv.value * 3)
+    faces = np.zeros(na.value * 3, np.int32)
+    fT, fw = np.zeros(nf.value), np.zeros(nf.value)
+    assert lib.dsmc_icosphere_build(levels, lon, 100.0, 135.0, mass, kB, ptr(vert, C.c_double),
+                                    ptr(faces, C.c_int32), ptr(fT, C.c_double), ptr(fw, C.c_double)) == 0
+    return SurfaceArrays(levels=levels, vert=vert, faces=faces, face_T=fT, face_w=fw)
+
+
+def test_device_math_bitwise(lib, dev):
+    """dm_* on gfx950 == the same functions on the host; sqrt and division are
+    correctly rounded (== numpy)."""
+    rng = np.random.default_rng(0)
+    n = 200000
+    cases = {0: (rng.uniform(-12, 12, n), None), 1: (rng.uniform(-12, 12, n), None),
+             2: (rng.uniform(-1, 1, n), None), 3: (rng.uniform(-5e5, 5e5, n), rng.uniform(-5e5, 5e5, n)),
+             4: (rng.uniform(-700, 700, n), None), 5: (np.exp(rng.uniform(-700, 700, n)), None)}
+    for fn, (x, y) in cases.items():
+        out = np.zeros(n)
+        yy = None if y is None else np.ascontiguousarray(y)
+        assert lib.dsmc_elementary(fn, ptr(x, C.c_double), ptr(yy, C.c_double), ptr(out, C.c_double), n) == 0
+        host = np.zeros(n)
+        dev.cpu_ref_elementary(fn, ptr(x, C.c_double), ptr(yy if yy is not None else x, C.c_double),
+                               ptr(host, C.c_double), n)
+        assert np.array_equal(out.view(np.int64), host.view(np.int64)), fn
+    x = rng.uniform(0, 1e14, n)
+    y = rng.uniform(1e-3, 1e7, n)
+    out = np.zeros(n)
+    assert lib.dsmc_elementary(6, ptr(x, C.c_double), None, ptr(out, C.c_double), n) == 0
+    assert np.array_equal(out, np.sqrt(x))
+    assert lib.dsmc_elementary(7, ptr(x, C.c_double), ptr(y, C.c_double), ptr(out, C.c_double), n) == 0
+    assert np.array_equal(out, x / y)
+
+
+def test_verlet_matches_reference_trace(lib):
+    """The device integrator reproduces consecutive states of the reference's
+    verlet_step trace (integrator.c:4-55) bit for bit."""
+    fx = golden("ballistic_keyed_T_thermal.npz")
+    tr = fx["trace"]
+    p = compat_params(36, 72, 1, 0)
+    GM = p.G * p.M
+    idx = [i for i in range(2, len(tr)) if tr[i - 2, 6] == 0 and tr[i - 1, 6] == 0]
+    st = np.zeros((len(idx), 9))
+    for j, i in enumerate(idx):
+        s2 = tr[i - 2, :3]
+        r = np.sqrt(s2[0] * s2[0] + s2[1] * s2[1] + s2[2] * s2[2])
+        st[j, :6] = tr[i - 1, :6]
+        st[j, 6:] = (GM / (r * r * r)) * s2
+    st = np.ascontiguousarray(st)
+    assert lib.dsmc_verlet_steps(ptr(st, C.c_double), len(idx), 1, GM, p.radius, p.max_height, p.dt, None) == 0
+    assert np.array_equal(st[:, :6], tr[idx, :6])
+
+
+@pytest.mark.parametrize("tname,n", [("T_thermal", 400), ("T_polar", 300)])
+def test_compat_ballistic_bitwise(lib, dev, tname, n):
+    T, ftd, ffd = read_T(GOLDEN / f"{tname}.dat")
+    p = compat_params(ftd, ffd, n, 11)
+    surf = SurfaceArrays(T, ftd, ffd)
+    got = Buffers(p, 5)
+    assert lib.dsmc_run_ballistic(C.byref(p), C.byref(surf.surface), C.byref(got.result)) == 0
+    want = run_oracle(dev, p, surf, nthreads=8)
+    assert np.array_equal(got.ttime, want.ttime)
+    assert np.array_equal(got.steps, want.steps)
+    assert np.array_equal(got.end_state, want.end_state)
+    assert np.array_equal(got.fate, want.fate)
+    assert np.array_equal(got.hops, want.hops)
+    assert got.counters() == want.counters()
+    assert np.allclose(got.density, want.density, rtol=1e-12, atol=1e-12)
+    assert np.allclose(got.horizon, want.horizon, rtol=1e-12, atol=1e-15)
+
+
+def test_physical_bitwise(lib, dev):
+    p = capi.preset(lib, PRESET_CERES_PHYSICAL, 36, 72, n=3000, seed=5)
+    surf = icosphere(lib, 3)
+    got = Buffers(p, 5, moments=True)
+    assert lib.dsmc_run_ballistic(C.byref(p), C.byref(surf.surface), C.byref(got.result)) == 0
+    want = run_oracle(dev, p, surf, nthreads=8, moments=True)
+    assert np.array_equal(got.steps, want.steps)
+    assert np.array_equal(got.end_state, want.end_state)
+    assert np.array_equal(got.fate, want.fate)
+    assert np.array_equal(got.hops, want.hops)
+    c = got.counters()
+    assert c == want.counters()
+    assert c["n_escaped"] > 0 and c["n_frozen"] > 0 and c["n_photolysed"] > 0 and c["n_capped"] > 0
+    assert np.array_equal(got.vcount, want.vcount)
+    assert np.allclose(got.density, want.density, rtol=1e-12, atol=0)
+    assert np.allclose(got.vmom, want.vmom, rtol=1e-9, atol=1e-6)
+
+
+def test_velocity_records_caps(lib):
+    """First-come velocity records (ballistic.c:215-232): at most 200 per
+    (td, ht), total within the cap, and every record's cell indices valid."""
+    T, ftd, ffd = read_T(GOLDEN / "T_thermal.dat")
+    p = compat_params(ftd, ffd, 3000, 2)
+    surf = SurfaceArrays(T, ftd, ffd)
+    got = Buffers(p, 5, vrec=True)
+    assert lib.dsmc_run_ballistic(C.byref(p), C.byref(surf.surface), C.byref(got.result)) == 0
+    n = got.result.vrec_n
+    assert 0 < n <= p.record_cap_total
+    td, fd, ht = got.vrec_td[:n], got.vrec_fd[:n], got.vrec_ht[:n]
+    assert td.min() >= 0 and td.max() < ftd and fd.min() >= 0 and fd.max() < ffd
+    assert ht.min() >= 0 and ht.max() < p.hd
+    cnt = np.bincount(td * p.hd + ht, minlength=ftd * p.hd)
+    assert cnt.max() == 200
+    v = got.vrec[:3 * n].reshape(-1, 3)
+    assert np.all(np.isfinite(v)) and np.all(np.linalg.norm(v, axis=1) < p.v_esc * 1.5)
+
+
+def test_staged_run_relaunch(lib):
+    """A staged run relaunched twice gives the same per-particle results."""
+    p = capi.preset(lib, PRESET_CERES_PHYSICAL, 36, 72, n=20000, seed=3)
+    surf = icosphere(lib, 3)
+    out = Buffers(p, 5, moments=True)
+    st = C.c_int(0)
+    run = lib.dsmc_run_create(C.byref(p), C.byref(surf.surface), C.byref(out.result), C.byref(st))
+    assert run and st.value == 0
+    res = []
+    for _ in range(2):
+        assert lib.dsmc_run_launch(run, None) == 0
+        assert lib.dsmc_run_sync(run) == 0
+        b = Buffers(p, 5, moments=True)
+        assert lib.dsmc_run_collect(run, C.byref(b.result)) == 0
+        assert lib.dsmc_run_kernel_ms(run) > 0
+        res.append(b)
+    lib.dsmc_run_destroy(run)
+    assert np.array_equal(res[0].steps, res[1].steps)
+    assert np.array_equal(res[0].end_state, res[1].end_state)
+    assert np.array_equal(res[0].vcount, res[1].vcount)
+    assert res[0].counters() == res[1].counters()
+
+
+def test_dropin_run_ballistic_files(lib, tmp_path):
+    """run_ballistic (ballistic.c:35) writes the four .dat files with the
+    reference's names and formats; their values agree with the in-memory API."""
+    tfile = str(GOLDEN / "T_thermal.dat")
+    prefix = str(tmp_path / "out")
+    code = (f"import ctypes as C, sys; sys.path.insert(0, {str(capi.PKG_DIR.parent)!r});"
+            f"from dsmc_suite_amd import capi; lib = capi.load();"
+            f"lib.run_ballistic(200, {tfile!r}.encode(), {prefix!r}.encode(), 4)")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120,
+                       env={**os.environ, "DSMC_SEED": "4"})
+    assert r.returncode == 0, r.stderr
+    assert "Starting ballistic simulation: 200 particles" in r.stdout
+    den = (tmp_path / "out_den.dat").read_text().split("\n")
+    assert den[:3] == ["36", "72", "100"] and len(den) == 3 + 100 * 36 * 72 + 1
+    tt = np.array((tmp_path / "out_ttime.dat").read_text().split(), dtype=float)
+    assert len(tt) == 5 * 200
+    T, ftd, ffd = read_T(GOLDEN / "T_thermal.dat")
+    p = compat_params(ftd, ffd, 200, 4)
+    got = Buffers(p, 5)
+    assert lib.dsmc_run_ballistic(C.byref(p), C.byref(SurfaceArrays(T, ftd, ffd).surface), C.byref(got.result)) == 0
+    assert np.array_equal(tt, np.array(["%9.5f" % v for v in got.ttime], dtype=float))
+    vr = (tmp_path / "out_v_record.dat").read_text().split("\n")
+    assert int(vr[0]) > 0
