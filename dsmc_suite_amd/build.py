@@ -22,7 +22,7 @@ BUILD = ROOT / "_build"
 LIB = PKG / "libdsmc_hip.so"
 ARCH = os.environ.get("DSMC_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels.hip", "host.hip"]
+HIP_SOURCES = ["kernels.hip", "host.hip", "advanced.hip"]
 CXX_SOURCES = ["io.cpp", "surface.cpp", "pending.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-munsafe-fp-atomics",

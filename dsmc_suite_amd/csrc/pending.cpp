@@ -13,11 +13,6 @@ static int pending(const char *what)
 }
 
 extern "C" {
-int dsmc_run_advanced(const dsmc_params *, const dsmc_surface *, const dsmc_phase2_input *, dsmc_result *)
-{ return pending("dsmc_run_advanced"); }
-int dsmc_build_fits(const dsmc_params *, const dsmc_phase2_input *, int32_t *, double *, double *, double *,
-                    int32_t *)
-{ return pending("dsmc_build_fits"); }
 int dsmc_run_batch(int, const dsmc_params *, const dsmc_surface *, dsmc_result *, const dsmc_launch_shape *)
 { return pending("dsmc_run_batch"); }
 int dsmc_autotune(const dsmc_params *, const dsmc_surface *, dsmc_launch_shape *)

@@ -185,3 +185,39 @@ def test_dropin_run_ballistic_files(lib, tmp_path):
     assert np.array_equal(tt, np.array(["%9.5f" % v for v in got.ttime], dtype=float))
     vr = (tmp_path / "out_v_record.dat").read_text().split("\n")
     assert int(vr[0]) > 0
+
+
+def test_fits_bitwise(lib, dev):
+    """Device binning (counting sort) + per-cell reverse-CDF polyfit ==
+    advanced.c:178-209 + :32-117 as restated by the oracle."""
+    from tests.helpers import phase2
+    from tests.test_library_cpu import oracle_fits
+    fx = golden("advanced_keyed.npz")
+    p = compat_params(36, 72, 1, 0, advanced=True)
+    inp, keep = phase2(fx)
+    cnt, co, mn, mx, va = oracle_fits(dev, p, inp)
+    n = len(cnt)
+    g_cnt, g_va = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    g_co, g_mn, g_mx = np.zeros(n * 4), np.zeros(n), np.zeros(n)
+    assert lib.dsmc_build_fits(C.byref(p), C.byref(inp), ptr(g_cnt, C.c_int32), ptr(g_co, C.c_double),
+                               ptr(g_mn, C.c_double), ptr(g_mx, C.c_double), ptr(g_va, C.c_int32)) == 0
+    assert np.array_equal(g_cnt, cnt) and np.array_equal(g_va, va) and va.sum() > 50
+    ok = va == 1
+    assert np.array_equal(g_co.reshape(-1, 4)[ok], co.reshape(-1, 4)[ok])
+    assert np.array_equal(g_mn[ok], mn[ok]) and np.array_equal(g_mx[ok], mx[ok])
+
+
+def test_compat_advanced_bitwise(lib, dev):
+    from tests.helpers import phase2
+    fx = golden("advanced_keyed.npz")
+    T, ftd, ffd = read_T(GOLDEN / "T_thermal.dat")
+    p = compat_params(ftd, ffd, 48, 9, advanced=True)
+    inp, keep = phase2(fx)
+    surf = SurfaceArrays(T, ftd, ffd)
+    got = Buffers(p, 9)
+    assert lib.dsmc_run_advanced(C.byref(p), C.byref(surf.surface), C.byref(inp), C.byref(got.result)) == 0
+    want = run_oracle(dev, p, surf, rows=9, phase2_input=inp, nthreads=8)
+    for k in ("ttime", "steps", "end_state", "fate", "hops"):
+        assert np.array_equal(getattr(got, k), getattr(want, k)), k
+    assert got.counters() == want.counters() and got.result.n_collisions > 100
+    assert np.allclose(got.density, want.density, rtol=1e-12, atol=1e-12)

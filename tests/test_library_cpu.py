@@ -219,3 +219,37 @@ def test_invalid_arguments_rejected_without_device(lib):
     b = Buffers(p, 5)
     assert lib.dsmc_params_preset(C.byref(p), 99, 36, 72) == capi.C.c_int(-1).value
     assert lib.dsmc_icosphere_sizes(11, None, None, None) == -1
+
+
+def oracle_fits(dev, p, inp):
+    ncell = p.ftd * p.ffd * p.hd
+    cnt = np.zeros(ncell, np.int32)
+    sp = np.zeros(ncell * p.max_per_cell)
+    dev.cpu_ref_bin_records(inp.vrec, inp.vrec_td, inp.vrec_fd, inp.vrec_ht, C.c_int64(inp.nrec), p.ftd, p.ffd,
+                            p.hd, p.max_per_cell, ptr(cnt, C.c_int32), ptr(sp, C.c_double))
+    co = np.zeros(ncell * (p.fit_degree + 1))
+    mn, mx, va = np.zeros(ncell), np.zeros(ncell), np.zeros(ncell, np.int32)
+    dev.cpu_ref_build_fits(ptr(cnt, C.c_int32), ptr(sp, C.c_double), p.ftd, p.ffd, p.hd, p.max_per_cell,
+                           p.fit_degree, p.min_fit_samples, ptr(co, C.c_double), ptr(mn, C.c_double),
+                           ptr(mx, C.c_double), ptr(va, C.c_int32), 4)
+    return cnt, co, mn, mx, va
+
+
+def test_lane_state_machine_matches_oracle_advanced(emu):
+    """Collisional phase (advanced.c:241-450), fits supplied from the oracle."""
+    from tests.helpers import golden, phase2
+    dev = oracle(devmath=True)
+    fx = golden("advanced_keyed.npz")
+    T, ftd, ffd = read_T(GOLDEN / "T_thermal.dat")
+    p = compat_params(ftd, ffd, 24, 9, advanced=True)
+    inp, keep = phase2(fx)
+    surf = SurfaceArrays(T, ftd, ffd)
+    want = run_oracle(dev, p, surf, rows=9, phase2_input=inp, nthreads=8)
+    cnt, co, mn, mx, va = oracle_fits(dev, p, inp)
+    got = Buffers(p, 9)
+    assert emu.emu_advanced(C.byref(p), C.byref(surf.surface), inp.copy_atm, C.c_int64(len(keep[0])),
+                            ptr(cnt, C.c_int32), ptr(co, C.c_double), ptr(mn, C.c_double), ptr(mx, C.c_double),
+                            ptr(va, C.c_int32), C.byref(got.result)) == 0
+    for k in ("ttime", "steps", "end_state", "fate", "hops"):
+        assert np.array_equal(getattr(got, k), getattr(want, k)), k
+    assert got.counters() == want.counters() and want.result.n_collisions > 100
