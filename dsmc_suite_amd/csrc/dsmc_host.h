@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include "../../include/dsmc_hip.h"
 #include "dsmc_tables.h"
+#include "dsmc_kernels.h"
+#include <vector>
 
 namespace dsmc {
 
@@ -31,13 +33,19 @@ struct RunDev {
     DevBuf vrec, vrec_i, count_nn, ctl;
     /* advanced */
     DevBuf atm, count_num, coeffs, pmin, pmax, pvalid, speeds;
+    /* particle records and slot queues */
+    DevBuf rec, flyq0, flyq1, svcq, qctl, counters, rec_total;
+    unsigned int *host_ctl = nullptr;     /* pinned readback of qctl */
+    int rounds = 0;
+    double flight_ms = 0.0;
+    ~RunDev();
 };
 
 int ensure_device();
 int device_props(int *cus);
 int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r, int advanced,
               RunDev *rd, hipStream_t st);
-int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st);
+int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int timed = 0);
 int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st);
 dsmc_launch_shape default_shape(const DsmcEnv &E);
 

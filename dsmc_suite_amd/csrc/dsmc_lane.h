@@ -41,7 +41,10 @@
 #define M_PI 3.14159265358979323846
 #endif
 
-enum { LANE_IDLE = 0, LANE_FLIGHT = 1, LANE_LANDED = 2, LANE_DONE = 3 };
+/* FLIGHT: in a hop; LANDED: hop ended on the surface (or the per-hop limit);
+ * ENDED: the particle's fate was decided during a step and its outputs are
+ * still to be written by the service; DONE: finished. */
+enum { LANE_IDLE = 0, LANE_FLIGHT = 1, LANE_LANDED = 2, LANE_DONE = 3, LANE_ENDED = 4 };
 enum { DSMC_VT_NT = 23, DSMC_VT_NV = 1000 };
 #define DSMC_FD_MARGIN 1e-12
 
@@ -91,6 +94,9 @@ struct DsmcEnv {
     unsigned long long *rec_total;
     int rec_cap_cell;
     int64_t rec_cap_total;
+    /* fate/step counters of this run: escaped, frozen, photolysed, depleted,
+     * capped, collisions, particle-steps (device kernels add here) */
+    unsigned long long *counters;
 };
 
 struct DsmcLane {
@@ -103,6 +109,21 @@ struct DsmcLane {
     uint64_t k, blk;
     uint32_t rb0, rb1, rb2, rb3;
     int phase, fate, hops, num_col, td, fd, ht, acc_cell, acc_n, rec_key, rec_full, rotated;
+};
+
+/* A particle between kernels: one 256-byte record per slot in device memory. */
+struct DsmcRec {
+    double sx, sy, sz, vx, vy, vz, ox, oy, oz;
+    double Ri, W, time, flight_time, thi, fi, rest_time, real_col_p, col_p, col_lo;
+    double lsx, lsy, lsz;
+    int64_t steps, n_day, n_photo, pid;
+    uint64_t k;
+    int32_t ii, phase, fate, hops, num_col, td, fd, ht, rotated, run;
+};
+
+/* Per-step lookup tables (the kernels keep a copy in LDS). */
+struct DsmcCellTabs {
+    const double *tq, *hr, *cs;
 };
 
 /* Atomic/plain accumulation, supplied by the caller (GPU atomics or host adds). */
@@ -177,6 +198,18 @@ DSMC_HD int fd_update(double x, double y, int fd, int ffd, const double *cs)
     double dhi = x * cs[2 * fd + 2] - y * cs[2 * fd + 3];
     if (dlo > eps && dhi < -eps) return fd;
     return fd_exact(x, y, ffd);
+}
+
+/* Latitude index from (z, Ri): kept while z/Ri is farther than 1e-12 (relative)
+ * from both thresholds of cell td (no division), else walked on q = z/Ri. */
+DSMC_HD int td_update(double z, double Ri, int td, int ftd, const double *tq)
+{
+    double eps = 1e-12 * Ri;
+    if (z - tq[td] * Ri > eps && tq[td + 1] * Ri - z > eps) return td;
+    double q = z / Ri;
+    while (td > 0 && q < tq[td]) td--;
+    while (td < ftd - 1 && q >= tq[td + 1]) td++;
+    return td;
 }
 
 /* Index k with tab[k] <= v < tab[k+1], starting from a nearby guess. */
@@ -277,6 +310,76 @@ DSMC_HD void lane_finish(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
     default: break;
     }
     L.phase = LANE_DONE;
+}
+
+/* Moves counts gathered in `cnt` to the run's device counters and clears them. */
+template <class Ops>
+DSMC_HD void counts_flush(DsmcCounts &cnt, const DsmcEnv &E)
+{
+    if (!E.counters) return;
+    unsigned long long v[7] = {cnt.esc, cnt.frozen, cnt.photo, cnt.depleted, cnt.capped, cnt.coll, cnt.steps};
+    for (int i = 0; i < 7; i++)
+        if (v[i]) Ops::add_u64(&E.counters[i], v[i]);
+    cnt.esc = cnt.frozen = cnt.photo = cnt.depleted = cnt.capped = cnt.coll = cnt.steps = 0;
+}
+
+/* ------------------------------------------------------------------ records */
+/* Only what a flight step reads or writes travels with the flight kernel. */
+template <int MODE, int ADV>
+DSMC_HD void rec_load_flight(DsmcLane &L, const DsmcRec &R)
+{
+    L.sx = R.sx; L.sy = R.sy; L.sz = R.sz;
+    L.vx = R.vx; L.vy = R.vy; L.vz = R.vz;
+    L.ox = R.ox; L.oy = R.oy; L.oz = R.oz;
+    L.Ri = R.Ri; L.W = R.W; L.steps = R.steps;
+    L.td = R.td; L.fd = R.fd; L.ht = R.ht;
+    L.phase = R.phase; L.fate = R.fate;
+    if (MODE == 0) { L.time = R.time; L.flight_time = R.flight_time; L.ii = R.ii; }
+    else { L.n_day = R.n_day; L.n_photo = R.n_photo; }
+    if (ADV) {
+        L.real_col_p = R.real_col_p; L.col_p = R.col_p; L.col_lo = R.col_lo;
+        L.num_col = R.num_col; L.k = R.k; L.pid = R.pid; L.blk = ~0ull;
+    }
+    L.acc_cell = -1; L.rec_key = -1; L.rec_full = 0;
+}
+
+template <int MODE, int ADV>
+DSMC_HD void rec_store_flight(const DsmcLane &L, DsmcRec &R)
+{
+    R.sx = L.sx; R.sy = L.sy; R.sz = L.sz;
+    R.vx = L.vx; R.vy = L.vy; R.vz = L.vz;
+    R.ox = L.ox; R.oy = L.oy; R.oz = L.oz;
+    R.Ri = L.Ri; R.W = L.W; R.steps = L.steps;
+    R.td = L.td; R.fd = L.fd; R.ht = L.ht;
+    R.phase = L.phase; R.fate = L.fate;
+    if (MODE == 0) { R.time = L.time; R.flight_time = L.flight_time; R.ii = (int32_t)L.ii; }
+    else { R.n_day = L.n_day; }
+    if (ADV) { R.real_col_p = L.real_col_p; R.num_col = L.num_col; R.k = L.k; }
+}
+
+DSMC_HD void rec_load_all(DsmcLane &L, const DsmcRec &R)
+{
+    L.sx = R.sx; L.sy = R.sy; L.sz = R.sz; L.vx = R.vx; L.vy = R.vy; L.vz = R.vz;
+    L.ox = R.ox; L.oy = R.oy; L.oz = R.oz;
+    L.Ri = R.Ri; L.W = R.W; L.time = R.time; L.flight_time = R.flight_time; L.thi = R.thi; L.fi = R.fi;
+    L.rest_time = R.rest_time; L.real_col_p = R.real_col_p; L.col_p = R.col_p; L.col_lo = R.col_lo;
+    L.lsx = R.lsx; L.lsy = R.lsy; L.lsz = R.lsz;
+    L.steps = R.steps; L.n_day = R.n_day; L.n_photo = R.n_photo; L.pid = R.pid; L.k = R.k;
+    L.ii = R.ii; L.phase = R.phase; L.fate = R.fate; L.hops = R.hops; L.num_col = R.num_col;
+    L.td = R.td; L.fd = R.fd; L.ht = R.ht; L.rotated = R.rotated;
+    L.blk = ~0ull; L.acc_cell = -1; L.rec_key = -1; L.rec_full = 0;
+}
+
+DSMC_HD void rec_store_all(const DsmcLane &L, DsmcRec &R, int32_t run)
+{
+    R.sx = L.sx; R.sy = L.sy; R.sz = L.sz; R.vx = L.vx; R.vy = L.vy; R.vz = L.vz;
+    R.ox = L.ox; R.oy = L.oy; R.oz = L.oz;
+    R.Ri = L.Ri; R.W = L.W; R.time = L.time; R.flight_time = L.flight_time; R.thi = L.thi; R.fi = L.fi;
+    R.rest_time = L.rest_time; R.real_col_p = L.real_col_p; R.col_p = L.col_p; R.col_lo = L.col_lo;
+    R.lsx = L.lsx; R.lsy = L.lsy; R.lsz = L.lsz;
+    R.steps = L.steps; R.n_day = L.n_day; R.n_photo = L.n_photo; R.pid = L.pid; R.k = L.k;
+    R.ii = (int32_t)L.ii; R.phase = L.phase; R.fate = L.fate; R.hops = L.hops; R.num_col = L.num_col;
+    R.td = L.td; R.fd = L.fd; R.ht = L.ht; R.rotated = L.rotated; R.run = run;
 }
 
 /* ------------------------------------------------------------------ hop start */
@@ -514,6 +617,7 @@ template <class Ops, int MODE, int ADV>
 DSMC_HD void lane_service(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
 {
     lane_flush<Ops>(L, E);
+    if (L.phase == LANE_ENDED) { lane_finish(L, E, cnt); return; }
     if (MODE == 0) {                         /* ballistic.c:239-256 / advanced.c:405-420 */
         double Ri = sqrt(L.sx * L.sx + L.sy * L.sy + L.sz * L.sz);
         pos_angles(L.sx, L.sy, L.sz, Ri, &L.thi, &L.fi);
@@ -576,7 +680,7 @@ DSMC_HD double sample_speed(const double *c, int deg, double min_v, double max_v
 /* integrator.c:4-55 followed by ballistic.c:195-236 / advanced.c:347-402, or the
  * physical-mode step.  Returns with L.phase = FLIGHT, LANDED or DONE. */
 template <class Ops, int MODE, int ADV>
-DSMC_HD void lane_step(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
+DSMC_HD void lane_step(DsmcLane &L, const DsmcEnv &E, const DsmcCellTabs &TB)
 {
     const double h = E.h;
     if (MODE == 0) { L.flight_time += h; L.time += h; }
@@ -601,29 +705,29 @@ DSMC_HD void lane_step(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
     double Rn = sqrt(nx * nx + ny * ny + nz * nz);
     L.Ri = Rn;
     if (Rn <= E.R) { L.phase = LANE_LANDED; return; }
-    if (Rn >= E.max_height) { lane_flush<Ops>(L, E); L.fate = 1; lane_finish(L, E, cnt); return; }
+    if (Rn >= E.max_height) { lane_flush<Ops>(L, E); L.fate = 1; L.phase = LANE_ENDED; return; }
 
     if (MODE != 0) {
         if (nx * E.sun_x + ny * E.sun_y > 0.0) {
             L.n_day++;
-            if (L.n_day >= L.n_photo) { lane_flush<Ops>(L, E); L.fate = 3; lane_finish(L, E, cnt); return; }
+            if (L.n_day >= L.n_photo) { lane_flush<Ops>(L, E); L.fate = 3; L.phase = LANE_ENDED; return; }
         }
-        L.td = table_walk(E.tq, L.td, E.ftd - 1, nz / Rn);
-        L.fd = fd_update(nx, ny, L.fd, E.ffd, E.cs);
-        L.ht = table_walk(E.hr, L.ht, E.hd, Rn);
+        L.td = td_update(nz, Rn, L.td, E.ftd, TB.tq);
+        L.fd = fd_update(nx, ny, L.fd, E.ffd, TB.cs);
+        L.ht = table_walk(TB.hr, L.ht, E.hd, Rn);
         if (L.ht < E.hd) lane_deposit<Ops>(L, E, (L.ht * E.ftd + L.td) * E.ffd + L.fd);
         int64_t cap = E.max_steps > 0 ? E.max_steps : INT64_MAX;
-        if (L.steps >= cap) { lane_flush<Ops>(L, E); L.fate = 5; lane_finish(L, E, cnt); }
+        if (L.steps >= cap) { lane_flush<Ops>(L, E); L.fate = 5; L.phase = LANE_ENDED; }
         return;
     }
 
-    L.td = table_walk(E.tq, L.td, E.ftd - 1, nz / Rn);
-    L.fd = fd_update(nx, ny, L.fd, E.ffd, E.cs);
+    L.td = td_update(nz, Rn, L.td, E.ftd, TB.tq);
+    L.fd = fd_update(nx, ny, L.fd, E.ffd, TB.cs);
     if (L.fd < E.ffd / 2) L.W *= E.decay;
     int64_t ii = L.ii++;
     if (ii + 1 >= E.max_hop_steps) L.phase = LANE_LANDED;   /* loop bound reached */
     if (ii == 0) return;
-    L.ht = table_walk(E.hr, L.ht, E.hd, Rn);
+    L.ht = table_walk(TB.hr, L.ht, E.hd, Rn);
     if (L.ht >= E.hd) return;
     if (ADV) {
         if (E.copy_atm && L.ht < E.hd / 3) {           /* advanced.c:369-398 */

@@ -168,25 +168,93 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
     } else {
         E.rec_cap_cell = 0;
     }
-    if ((rc = rd->ctl.zeros(sizeof(unsigned long long) * 16, st))) return rc;
-    E.rec_total = (unsigned long long *)rd->ctl.ptr + 8;
+    if ((rc = rd->rec_total.zeros(sizeof(unsigned long long), st))) return rc;
+    if ((rc = rd->counters.zeros(sizeof(unsigned long long) * 8, st))) return rc;
+    E.rec_total = (unsigned long long *)rd->rec_total.ptr;
+    E.counters = (unsigned long long *)rd->counters.ptr;
+    size_t ns = (size_t)(n > 0 ? n : 1);
+    if (n > INT32_MAX) return DSMC_ERR_ARG;
+    if ((rc = rd->rec.alloc(ns * sizeof(DsmcRec)))) return rc;
+    if ((rc = rd->flyq0.alloc(ns * sizeof(int32_t)))) return rc;
+    if ((rc = rd->flyq1.alloc(ns * sizeof(int32_t)))) return rc;
+    if ((rc = rd->svcq.alloc(ns * sizeof(int32_t)))) return rc;
+    if ((rc = rd->qctl.zeros(sizeof(unsigned int) * 16, st))) return rc;
+    if (!rd->host_ctl && hipHostMalloc((void **)&rd->host_ctl, sizeof(unsigned int) * 16, 0) != hipSuccess)
+        return DSMC_ERR_NOMEM;
     return DSMC_OK;
 }
 
-int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st)
+RunDev::~RunDev()
+{
+    if (host_ctl) (void)hipHostFree(host_ctl);
+}
+
+/* Runs a staged run to completion on stream st: the start kernel, then rounds of
+ * flight + service until no particle is in flight.  Completion is checked on the
+ * host every `chunk` rounds (doubling up to 128), so a run costs few syncs. */
+int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int timed)
 {
     int cus = 256;
     int rc = device_props(&cus);
     if (rc) return rc;
     dsmc_launch_shape sh = shape ? *shape : default_shape(rd->E);
-    if (sh.block <= 0 || sh.block % 64 || sh.block > 1024 || sh.blocks_per_cu <= 0) return DSMC_ERR_ARG;
-    int64_t waves_needed = (rd->E.nparticles + 63) / 64;
-    int64_t grid = (int64_t)cus * sh.blocks_per_cu;
-    int64_t max_grid = (waves_needed * 64 + sh.block - 1) / sh.block;
-    if (grid > max_grid) grid = max_grid;
-    if (grid < 1) grid = 1;
-    unsigned long long *ctl = (unsigned long long *)rd->ctl.ptr;
-    HIP_OK(dsmc_launch_particles(&rd->E, ctl, ctl + 1, (int)grid, sh.block, sh.relaunch_batch, st));
+    if (sh.block != 256 || sh.blocks_per_cu <= 0 || sh.blocks_per_cu > 64 || sh.relaunch_batch < 1 ||
+        sh.relaunch_batch > 64)
+        return DSMC_ERR_ARG;
+    const int64_t n = rd->E.nparticles;
+    DsmcQueues Q;
+    Q.fly[0] = (int32_t *)rd->flyq0.ptr;
+    Q.fly[1] = (int32_t *)rd->flyq1.ptr;
+    Q.svc = (int32_t *)rd->svcq.ptr;
+    Q.ctl = (unsigned int *)rd->qctl.ptr;
+    DsmcRec *rec = (DsmcRec *)rd->rec.ptr;
+    HIP_OK(hipMemsetAsync(Q.ctl, 0, sizeof(unsigned int) * 16, st));
+    rd->rounds = 0;
+    rd->flight_ms = 0.0;
+    if (n == 0) return DSMC_OK;
+    int64_t blocks = (n + 255) / 256;
+    int start_grid = (int)(blocks < (int64_t)cus * 8 ? blocks : (int64_t)cus * 8);
+    HIP_OK(dsmc_launch_start(&rd->E, nullptr, nullptr, 1, rec, 0, n, Q, 0, start_grid, st));
+    int flight_grid = cus * sh.blocks_per_cu;
+    if ((int64_t)flight_grid > blocks) flight_grid = (int)blocks;
+    int service_grid = start_grid;
+    /* per-launch step budget: long hops are split over rounds so the run's tail is
+     * set by its longest particle, not by the sum of each round's longest hop */
+    const int64_t sh_budget = rd->E.mode == 0 ? 4096 : 1024;
+    std::vector<hipEvent_t> ev;
+    int p = 0, chunk = 4;
+    for (;;) {
+        for (int r = 0; r < chunk; r++) {
+            if (timed) {
+                hipEvent_t a, b;
+                HIP_OK(hipEventCreate(&a));
+                HIP_OK(hipEventCreate(&b));
+                ev.push_back(a);
+                ev.push_back(b);
+                HIP_OK(hipEventRecord(a, st));
+            }
+            HIP_OK(dsmc_launch_flight(&rd->E, nullptr, rec, Q, p, flight_grid, sh.relaunch_batch,
+                                      sh_budget, st));
+            if (timed) HIP_OK(hipEventRecord(ev.back(), st));
+            HIP_OK(dsmc_launch_service(&rd->E, nullptr, rec, Q, p, service_grid, st));
+            p ^= 1;
+        }
+        HIP_OK(hipMemcpyAsync(rd->host_ctl, Q.ctl, sizeof(unsigned int) * 16, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        if (rd->host_ctl[2 * p] == 0) break;
+        if (rd->host_ctl[8] > (1u << 24)) {
+            fprintf(stderr, "dsmc_hip: run did not finish within %u rounds\n", rd->host_ctl[8]);
+            return DSMC_ERR_DEVICE;
+        }
+        if (chunk < 128) chunk *= 2;
+    }
+    rd->rounds = (int)rd->host_ctl[8];
+    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+        rd->flight_ms += ms;
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     return DSMC_OK;
 }
 
@@ -194,8 +262,8 @@ dsmc_launch_shape default_shape(const DsmcEnv &E)
 {
     dsmc_launch_shape s;
     s.block = 256;
-    s.blocks_per_cu = 2;
-    s.relaunch_batch = E.mode == 0 ? 24 : 16;
+    s.blocks_per_cu = 4;
+    s.relaunch_batch = E.mode == 0 ? 8 : 16;
     return s;
 }
 
@@ -203,8 +271,9 @@ int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st
 {
     const size_t cells = (size_t)p->hd * p->ftd * p->ffd;
     const int64_t n = p->nparticles;
-    unsigned long long ctl[16];
-    HIP_OK(hipMemcpyAsync(ctl, rd->ctl.ptr, sizeof ctl, hipMemcpyDeviceToHost, st));
+    unsigned long long ctl[8], rtot = 0;
+    HIP_OK(hipMemcpyAsync(ctl, rd->counters.ptr, sizeof ctl, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(&rtot, rd->rec_total.ptr, sizeof rtot, hipMemcpyDeviceToHost, st));
     if (r->density) HIP_OK(hipMemcpyAsync(r->density, rd->density.ptr, cells * 8, hipMemcpyDeviceToHost, st));
     if (r->horizon)
         HIP_OK(hipMemcpyAsync(r->horizon, rd->horizon.ptr, sizeof(double) * p->ftd * p->ffd,
@@ -223,7 +292,7 @@ int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st
     HIP_OK(hipStreamSynchronize(st));
     r->vrec_n = 0;
     if (rd->E.vrec) {
-        int64_t nrec = (int64_t)ctl[8];
+        int64_t nrec = (int64_t)rtot;
         if (nrec > p->record_cap_total) nrec = p->record_cap_total;
         size_t cap = (size_t)p->record_cap_total;
         if (nrec > 0) {
@@ -236,13 +305,13 @@ int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st
         r->vrec_n = nrec;
     }
     if (r->ttime && !rd->ttime.ptr) memset(r->ttime, 0, sizeof(double) * 5 * n);
-    r->n_escaped = (int64_t)ctl[1];
-    r->n_frozen = (int64_t)ctl[2];
-    r->n_photolysed = (int64_t)ctl[3];
-    r->n_depleted = (int64_t)ctl[4];
-    r->n_capped = (int64_t)ctl[5];
-    r->n_collisions = (int64_t)ctl[6];
-    r->total_steps = (int64_t)ctl[7];
+    r->n_escaped = (int64_t)ctl[0];
+    r->n_frozen = (int64_t)ctl[1];
+    r->n_photolysed = (int64_t)ctl[2];
+    r->n_depleted = (int64_t)ctl[3];
+    r->n_capped = (int64_t)ctl[4];
+    r->n_collisions = (int64_t)ctl[5];
+    r->total_steps = (int64_t)ctl[6];
     return DSMC_OK;
 }
 
@@ -396,7 +465,7 @@ extern "C" dsmc_run *dsmc_run_create(const dsmc_params *p, const dsmc_surface *s
 static int clear_outputs(RunDev &rd, hipStream_t st)
 {
     DevBuf *bufs[] = {&rd.density, &rd.horizon, &rd.vmom, &rd.vcount, &rd.ttime, &rd.end_state,
-                      &rd.steps, &rd.fate, &rd.hops, &rd.count_nn, &rd.ctl};
+                      &rd.steps, &rd.fate, &rd.hops, &rd.count_nn, &rd.counters, &rd.rec_total};
     for (DevBuf *b : bufs)
         if (b->ptr) HIP_OK(hipMemsetAsync(b->ptr, 0, b->bytes, st));
     return DSMC_OK;
@@ -408,7 +477,7 @@ extern "C" int dsmc_run_launch(dsmc_run *run, const dsmc_launch_shape *shape)
     int rc = clear_outputs(run->rd, run->st);
     if (rc) return rc;
     HIP_OK(hipEventRecord(run->e0, run->st));
-    rc = launch_run(&run->rd, shape, run->st);
+    rc = launch_run(&run->rd, shape, run->st, 1);
     if (rc) return rc;
     HIP_OK(hipEventRecord(run->e1, run->st));
     run->launched = 1;
@@ -430,6 +499,14 @@ extern "C" double dsmc_run_kernel_ms(dsmc_run *run)
     return ms;
 }
 
+extern "C" int dsmc_run_timing(dsmc_run *run, double *flight_ms, int *rounds)
+{
+    if (!run || !run->launched) return DSMC_ERR_ARG;
+    if (flight_ms) *flight_ms = run->rd.flight_ms;
+    if (rounds) *rounds = run->rd.rounds;
+    return DSMC_OK;
+}
+
 extern "C" int dsmc_run_collect(dsmc_run *run, dsmc_result *r)
 {
     if (!run || !r) return DSMC_ERR_ARG;
@@ -444,10 +521,9 @@ extern "C" void dsmc_run_destroy(dsmc_run *run)
     if (run->st) (void)hipStreamSynchronize(run->st);
     if (run->e0) (void)hipEventDestroy(run->e0);
     if (run->e1) (void)hipEventDestroy(run->e1);
-    run->rd.~RunDev();
-    new (&run->rd) RunDev();
-    if (run->st) (void)hipStreamDestroy(run->st);
+    hipStream_t st = run->st;
     delete run;
+    if (st) (void)hipStreamDestroy(st);
 }
 
 /* ------------------------------------------------------------------ diagnostics */

@@ -1,20 +1,32 @@
 /*
- * kernels.hip - gfx950 kernels of the DSMC path.
+ * kernels.hip - gfx950 kernels of the DSMC path (event-based).
  *
- * dsmc_particles_kernel is persistent: each 64-lane wavefront owns 64 particles
- * at a time and pulls new particle ids from a global counter (one atomic per
- * wave per refill).  The flight step (Stormer-Verlet + cell tracking + deposit,
- * FP64 VALU only) runs for every flying lane; lanes that land wait, masked, until
- * enough of the wave has landed, and then the divergent relaunch work (angles,
- * frozen scan, velocity sampling, trigonometry) runs once for all of them.  The
- * per-lane logic is dsmc_lane.h; this file only schedules it.
+ * A particle lives in a 256-byte record (DsmcRec) in device memory between
+ * kernels.  One "round" of a run is three launches on one stream:
  *
- * Deposits are accumulated in registers while a particle stays in one cell and
- * merged with one global FP64 atomic per cell change (the grid is L2-resident).
+ *   flight   persistent; each wavefront pulls particle slots from fly queue p
+ *            (one atomic per refill of many lanes), loads only the fields a
+ *            Stormer-Verlet step touches, steps every lane until its hop ends
+ *            on the surface or its fate is decided, flushes its register deposit
+ *            accumulator, stores the record and appends the slot to the service
+ *            queue.  Lanes that used up a per-launch step budget go to fly queue
+ *            1-p instead (time slicing for the serving loop).
+ *   service  one lane per queued slot: landing bookkeeping and relaunch (angles,
+ *            frozen scan, emission, trigonometry), or the particle's outputs;
+ *            relaunched particles are appended to fly queue 1-p.
+ *   reset    clears the consumed queues' counters (a kernel, so rounds can be
+ *            captured in a hipGraph and replayed).
+ *
+ * The flight kernel holds no relaunch code, which keeps its register footprint -
+ * and so its occupancy - set by the Verlet step alone.  Every particle's
+ * arithmetic is the per-lane state machine of dsmc_lane.h; runs of a batch share
+ * the queues and carry their own environment (BATCH template).
  */
 #include <hip/hip_runtime.h>
 #include "dsmc_lane.h"
 #include "dsmc_kernels.h"
+
+static_assert(sizeof(DsmcRec) == 256, "particle record must stay 256 bytes");
 
 struct DsmcOpsDev {
     __device__ static inline void add_f64(double *p, double v) { atomicAdd(p, v); }
@@ -29,64 +41,165 @@ __device__ static inline uint64_t lanemask_lt()
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-template <int MODE, int ADV>
-__global__ void __launch_bounds__(256)
-dsmc_particles_kernel(DsmcEnv E, unsigned long long *next_pid, unsigned long long *counters,
-                      int relaunch_batch)
+/* Wave-aggregated append of `slot` to queue q (counter *cnt) for lanes with pred. */
+__device__ static inline void wave_append(bool pred, int32_t slot, int32_t *q, unsigned int *cnt)
 {
+    uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    int leader = __ffsll((long long)m) - 1;
+    unsigned int base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(cnt, (unsigned int)__popcll(m));
+    base = __shfl(base, leader);
+    if (pred) q[base + __popcll(m & lanemask_lt())] = slot;
+}
+
+template <bool BATCH>
+__device__ static inline const DsmcEnv &env_of(const DsmcEnv &E0, const DsmcEnv *envs, int run)
+{
+    return BATCH ? envs[run] : E0;
+}
+
+/* ------------------------------------------------------------------ start */
+/* Initialises slots [0, nslots): slot -> (run, pid) through run_base. */
+template <int MODE, int ADV, bool BATCH>
+__global__ void __launch_bounds__(256)
+start_kernel(DsmcEnv E0, const DsmcEnv *envs, const int64_t *run_base, int nruns, DsmcRec *rec,
+             int64_t slot0, int64_t nslots, int32_t *flyq, unsigned int *fly_count)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < nslots; b += stride) {
+        int64_t i = b + __lane_id();
+        bool fly = false;
+        int64_t slot = slot0 + i;
+        if (i < nslots) {
+            int run = 0;
+            if (BATCH) while (run + 1 < nruns && slot >= run_base[run + 1]) run++;
+            const DsmcEnv &E = env_of<BATCH>(E0, envs, run);
+            DsmcLane L;
+            DsmcCounts cnt = {0, 0, 0, 0, 0, 0, 0};
+            lane_start<DsmcOpsDev, MODE, ADV>(L, E, slot - (BATCH ? run_base[run] : slot0), cnt);
+            counts_flush<DsmcOpsDev>(cnt, E);
+            fly = L.phase == LANE_FLIGHT;
+            if (fly) rec_store_all(L, rec[slot], run);
+        }
+        wave_append(fly, (int32_t)slot, flyq, fly_count);
+    }
+}
+
+/* ------------------------------------------------------------------ flight */
+template <int MODE, int ADV, bool BATCH>
+__global__ void __launch_bounds__(256)
+flight_kernel(DsmcEnv E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch,
+              int64_t step_budget)
+{
+    /* per-step tables -> LDS (single-run launches; a batch reads each run's own) */
+    extern __shared__ double lds_tabs[];
+    DsmcCellTabs TB;
+    if (!BATCH) {
+        const int ntq = E0.ftd + 1, nhr = E0.hd + 2, ncs = 2 * (E0.ffd + 1);
+        for (int i = threadIdx.x; i < ntq + nhr + ncs; i += blockDim.x)
+            lds_tabs[i] = i < ntq ? E0.tq[i] : (i < ntq + nhr ? E0.hr[i - ntq] : E0.cs[i - ntq - nhr]);
+        __syncthreads();
+        TB.tq = lds_tabs;
+        TB.hr = lds_tabs + ntq;
+        TB.cs = lds_tabs + ntq + nhr;
+    }
+    const unsigned int nq = Q.ctl[2 * p];           /* fly queue p is not appended to in this launch */
+    unsigned int *head = &Q.ctl[2 * p + 1];
+    const int32_t *flyq = Q.fly[p];
     const uint32_t lane = __lane_id();
-    const uint64_t ltmask = lanemask_lt();
     DsmcLane L;
     L.phase = LANE_IDLE;
-    DsmcCounts cnt = {0, 0, 0, 0, 0, 0, 0};
+    int32_t slot = -1, run = 0;
+    int64_t budget_end = 0;
     bool exhausted = false;
-    const int64_t N = E.nparticles;
-
     for (;;) {
-        /* refill lanes without a particle (wave-aggregated fetch) */
-        while (!exhausted) {
-            uint64_t need = __ballot(L.phase == LANE_DONE || L.phase == LANE_IDLE);
-            if (need == 0) break;
-            int nneed = __popcll(need);
-            int leader = __ffsll((long long)need) - 1;
-            unsigned long long base = 0;
-            if ((int)lane == leader) base = atomicAdd(next_pid, (unsigned long long)nneed);
+        /* refill idle lanes together (one atomic per wave) */
+        uint64_t idle = __ballot(L.phase == LANE_IDLE);
+        uint64_t flying = ~idle & __ballot(true);
+        if (!exhausted && idle && (__popcll(idle) >= refill_batch || flying == 0)) {
+            int leader = __ffsll((long long)idle) - 1;
+            unsigned int base = 0;
+            if ((int)lane == leader) base = atomicAdd(head, (unsigned int)__popcll(idle));
             base = __shfl(base, leader);
-            if ((int64_t)(base + nneed) >= N) exhausted = true;
-            if (need & (1ull << lane)) {
-                int64_t pid = (int64_t)base + __popcll(need & ltmask);
-                if (pid < N) lane_start<DsmcOpsDev, MODE, ADV>(L, E, pid, cnt);
-                else L.phase = LANE_IDLE;
+            if (base + (unsigned int)__popcll(idle) >= nq) exhausted = true;
+            if (idle & (1ull << lane)) {
+                unsigned int qi = base + __popcll(idle & lanemask_lt());
+                if (qi < nq) {
+                    slot = flyq[qi];
+                    if (BATCH) run = rec[slot].run;
+                    rec_load_flight<MODE, ADV>(L, rec[slot]);
+                    budget_end = step_budget > 0 ? L.steps + step_budget : INT64_MAX;
+                }
             }
+            flying = __ballot(L.phase == LANE_FLIGHT);
         }
-        uint64_t flying = __ballot(L.phase == LANE_FLIGHT);
-        uint64_t landed = __ballot(L.phase == LANE_LANDED);
-        if (flying == 0 && landed == 0) {
+        if (flying == 0) {
             if (exhausted) break;
             continue;
         }
-        /* flight: step until enough lanes wait for service */
-        int active = __popcll(__ballot(L.phase != LANE_IDLE && L.phase != LANE_DONE));
-        int target = active / 2 < relaunch_batch ? active / 2 : relaunch_batch;
-        if (target < 1) target = 1;
-        while (flying != 0) {
-            uint64_t waiting = landed | (exhausted ? 0ull : __ballot(L.phase == LANE_DONE));
-            if (__popcll(waiting) >= target) break;
-            if (L.phase == LANE_FLIGHT) lane_step<DsmcOpsDev, MODE, ADV>(L, E, cnt);
-            flying = __ballot(L.phase == LANE_FLIGHT);
-            landed = __ballot(L.phase == LANE_LANDED);
+        /* step until some lanes have ended their hop */
+        const DsmcEnv &E = env_of<BATCH>(E0, envs, run);
+        if (BATCH) { TB.tq = E.tq; TB.hr = E.hr; TB.cs = E.cs; }
+        for (;;) {
+            if (L.phase == LANE_FLIGHT) {
+                lane_step<DsmcOpsDev, MODE, ADV>(L, E, TB);
+                if (L.phase == LANE_FLIGHT && L.steps >= budget_end) L.phase = LANE_IDLE + 8;
+            }
+            uint64_t ended = __ballot(L.phase != LANE_FLIGHT && L.phase != LANE_IDLE);
+            if (ended && (__popcll(ended | idle) >= refill_batch || __ballot(L.phase == LANE_FLIGHT) == 0)) break;
         }
-        /* service: landing bookkeeping + relaunch for all waiting lanes */
-        if (L.phase == LANE_LANDED) lane_service<DsmcOpsDev, MODE, ADV>(L, E, cnt);
+        /* hand ended hops to the service queue, budget-stopped lanes to the next round */
+        bool hop_end = L.phase == LANE_LANDED || L.phase == LANE_ENDED;
+        bool paused = L.phase == LANE_IDLE + 8;
+        if (hop_end || paused) {
+            lane_flush<DsmcOpsDev>(L, E);
+            if (paused) L.phase = LANE_FLIGHT;
+            rec_store_flight<MODE, ADV>(L, rec[slot]);
+        }
+        wave_append(hop_end, slot, Q.svc, &Q.ctl[4]);
+        wave_append(paused, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
+        if (hop_end || paused) L.phase = LANE_IDLE;
     }
-    /* fate and step counters: one atomic per counter per lane with nonzero value */
-    if (cnt.esc) atomicAdd(&counters[0], cnt.esc);
-    if (cnt.frozen) atomicAdd(&counters[1], cnt.frozen);
-    if (cnt.photo) atomicAdd(&counters[2], cnt.photo);
-    if (cnt.depleted) atomicAdd(&counters[3], cnt.depleted);
-    if (cnt.capped) atomicAdd(&counters[4], cnt.capped);
-    if (cnt.coll) atomicAdd(&counters[5], cnt.coll);
-    if (cnt.steps) atomicAdd(&counters[6], cnt.steps);
+}
+
+/* ------------------------------------------------------------------ service */
+template <int MODE, int ADV, bool BATCH>
+__global__ void __launch_bounds__(256)
+service_kernel(DsmcEnv E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int p)
+{
+    const unsigned int n = Q.ctl[4];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < (int64_t)n; b += stride) {
+        int64_t i = b + __lane_id();
+        bool fly = false;
+        int32_t slot = 0;
+        if (i < n) {
+            slot = Q.svc[i];
+            int run = BATCH ? rec[slot].run : 0;
+            const DsmcEnv &E = env_of<BATCH>(E0, envs, run);
+            DsmcLane L;
+            rec_load_all(L, rec[slot]);
+            DsmcCounts cnt = {0, 0, 0, 0, 0, 0, 0};
+            lane_service<DsmcOpsDev, MODE, ADV>(L, E, cnt);
+            counts_flush<DsmcOpsDev>(cnt, E);
+            fly = L.phase == LANE_FLIGHT;
+            if (fly) rec_store_all(L, rec[slot], run);
+        }
+        wave_append(fly, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
+    }
+}
+
+/* ------------------------------------------------------------------ reset */
+__global__ void reset_kernel(unsigned int *ctl, int p, unsigned int *rounds)
+{
+    if (threadIdx.x == 0) {
+        ctl[2 * p] = 0;
+        ctl[2 * p + 1] = 0;
+        ctl[4] = 0;
+        ctl[5] = 0;
+        rounds[0] += 1;
+    }
 }
 
 /* Plain FP64 Verlet steps of independent states (integrator.c:4-55), for the
@@ -140,20 +253,44 @@ __global__ void dsmc_math_kernel(int fn, const double *x, const double *y, doubl
 }
 
 /* ------------------------------------------------------------------ launchers */
-extern "C" hipError_t dsmc_launch_particles(const DsmcEnv *E, unsigned long long *next_pid,
-                                            unsigned long long *counters, int grid, int block,
-                                            int relaunch_batch, hipStream_t stream)
+#define DISPATCH(KERNEL, ...)                                                                  \
+    do {                                                                                       \
+        if (mode == 0 && !adv && !batch) hipLaunchKernelGGL((KERNEL<0, 0, false>), __VA_ARGS__); \
+        else if (mode == 0 && !adv) hipLaunchKernelGGL((KERNEL<0, 0, true>), __VA_ARGS__);     \
+        else if (mode == 0 && !batch) hipLaunchKernelGGL((KERNEL<0, 1, false>), __VA_ARGS__);  \
+        else if (mode == 0) hipLaunchKernelGGL((KERNEL<0, 1, true>), __VA_ARGS__);             \
+        else if (!batch) hipLaunchKernelGGL((KERNEL<1, 0, false>), __VA_ARGS__);               \
+        else hipLaunchKernelGGL((KERNEL<1, 0, true>), __VA_ARGS__);                            \
+    } while (0)
+
+extern "C" hipError_t dsmc_launch_start(const DsmcEnv *E0, const DsmcEnv *envs, const int64_t *run_base,
+                                        int nruns, DsmcRec *rec, int64_t slot0, int64_t nslots,
+                                        DsmcQueues Q, int p, int grid, hipStream_t st)
 {
-    const size_t lds = 0;
-    if (E->mode == 0 && !E->advanced)
-        hipLaunchKernelGGL((dsmc_particles_kernel<0, 0>), dim3(grid), dim3(block), lds, stream, *E,
-                           next_pid, counters, relaunch_batch);
-    else if (E->mode == 0)
-        hipLaunchKernelGGL((dsmc_particles_kernel<0, 1>), dim3(grid), dim3(block), lds, stream, *E,
-                           next_pid, counters, relaunch_batch);
-    else
-        hipLaunchKernelGGL((dsmc_particles_kernel<1, 0>), dim3(grid), dim3(block), lds, stream, *E,
-                           next_pid, counters, relaunch_batch);
+    const int mode = E0->mode, adv = E0->advanced;
+    const bool batch = envs != nullptr;
+    DISPATCH(start_kernel, dim3(grid), dim3(256), 0, st, *E0, envs, run_base, nruns, rec, slot0, nslots,
+             Q.fly[p], &Q.ctl[2 * p]);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t dsmc_launch_flight(const DsmcEnv *E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q,
+                                         int p, int grid, int refill_batch, int64_t step_budget, hipStream_t st)
+{
+    const int mode = E0->mode, adv = E0->advanced;
+    const bool batch = envs != nullptr;
+    const size_t lds = batch ? 0 : sizeof(double) * (size_t)((E0->ftd + 1) + (E0->hd + 2) + 2 * (E0->ffd + 1));
+    DISPATCH(flight_kernel, dim3(grid), dim3(256), lds, st, *E0, envs, rec, Q, p, refill_batch, step_budget);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t dsmc_launch_service(const DsmcEnv *E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q,
+                                          int p, int grid, hipStream_t st)
+{
+    const int mode = E0->mode, adv = E0->advanced;
+    const bool batch = envs != nullptr;
+    DISPATCH(service_kernel, dim3(grid), dim3(256), 0, st, *E0, envs, rec, Q, p);
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, Q.ctl, p, Q.ctl + 8);
     return hipGetLastError();
 }
 

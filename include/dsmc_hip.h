@@ -195,9 +195,9 @@ int dsmc_run_advanced(const dsmc_params *p, const dsmc_surface *s,
 
 /* Launch shape of the persistent particle kernel. */
 typedef struct dsmc_launch_shape {
-    int block;                /* threads per block (multiple of 64) */
-    int blocks_per_cu;        /* resident blocks per CU */
-    int relaunch_batch;       /* lanes that must land before a wave relaunches them */
+    int block;                /* threads per block of the flight kernel (256) */
+    int blocks_per_cu;        /* persistent flight blocks per CU */
+    int relaunch_batch;       /* idle lanes a wavefront gathers before pulling new particles */
 } dsmc_launch_shape;
 
 /* Staged runs: inputs uploaded once and kept resident in device memory, the run
@@ -209,6 +209,8 @@ dsmc_run *dsmc_run_create(const dsmc_params *p, const dsmc_surface *s, const dsm
 int dsmc_run_launch(dsmc_run *run, const dsmc_launch_shape *shape);  /* async on the run's stream */
 int dsmc_run_sync(dsmc_run *run);
 double dsmc_run_kernel_ms(dsmc_run *run);        /* HIP-event time of the last launch */
+/* Device time spent in the flight kernel and number of rounds of the last launch. */
+int dsmc_run_timing(dsmc_run *run, double *flight_ms, int *rounds);
 int dsmc_run_collect(dsmc_run *run, dsmc_result *r);
 void dsmc_run_destroy(dsmc_run *run);
 

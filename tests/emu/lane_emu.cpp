@@ -13,8 +13,9 @@ template <int MODE, int ADV>
 static void run_one(DsmcLane &L, const DsmcEnv &E, int64_t n, DsmcCounts &cnt)
 {
     lane_start<DsmcOpsHost, MODE, ADV>(L, E, n, cnt);
+    DsmcCellTabs TB = {E.tq, E.hr, E.cs};
     while (L.phase != LANE_DONE) {
-        if (L.phase == LANE_FLIGHT) lane_step<DsmcOpsHost, MODE, ADV>(L, E, cnt);
+        if (L.phase == LANE_FLIGHT) lane_step<DsmcOpsHost, MODE, ADV>(L, E, TB);
         else lane_service<DsmcOpsHost, MODE, ADV>(L, E, cnt);
     }
 }
