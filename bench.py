@@ -82,6 +82,7 @@ def main():
     ap.add_argument("--relaunch-batch", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=60000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-moments", action="store_true", help="density only (no velocity moments)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -91,7 +92,7 @@ def main():
     lib = capi.load()
     p = capi.preset(lib, PRESET_CERES_PHYSICAL, 36, 72, n=args.particles, seed=args.seed)
     surf = icosphere(lib, 3)
-    outs = Buffers(p, 5, per_particle=False, moments=True)
+    outs = Buffers(p, 5, per_particle=False, moments=not args.no_moments)
     st = C.c_int(0)
     run = lib.dsmc_run_create(C.byref(p), C.byref(surf.surface), C.byref(outs.result), C.byref(st))
     if not run:

@@ -82,6 +82,7 @@ struct DsmcEnv {
     const double *pmin, *pmax;
     const int32_t *pvalid;
     /* outputs */
+    int density_from_count;   /* physical mode with counts: W == 1, density = vcount */
     double *density, *horizon, *vmom;
     unsigned long long *vcount;
     double *ttime, *end_state;
@@ -128,6 +129,16 @@ struct DsmcCellTabs {
 
 /* Atomic/plain accumulation, supplied by the caller (GPU atomics or host adds). */
 struct DsmcOpsHost {
+    static inline void deposit(const DsmcEnv &E, int cell, double w, double vx, double vy, double vz, int n)
+    {
+        if (!E.density_from_count) E.density[cell] += w;
+        if (E.vmom) {
+            E.vmom[(size_t)cell * 3 + 0] += vx;
+            E.vmom[(size_t)cell * 3 + 1] += vy;
+            E.vmom[(size_t)cell * 3 + 2] += vz;
+        }
+        if (E.vcount) E.vcount[cell] += (unsigned long long)n;
+    }
     static inline void add_f64(double *p, double v) { *p += v; }
     static inline void add_u64(unsigned long long *p, unsigned long long v) { *p += v; }
     static inline int32_t claim_i32(int32_t *p) { return (*p)++; }
@@ -179,6 +190,11 @@ DSMC_HD void pos_angles(double x, double y, double z, double Ri, double *thi, do
     if (*fi < 0) *fi += 2.0 * M_PI;
 }
 
+/* Out of line on the device: it runs only within 1e-12 of a sector boundary, and
+ * inlined it would set the register footprint of the whole flight kernel. */
+#if defined(__HIP_DEVICE_COMPILE__)
+__attribute__((noinline))
+#endif
 DSMC_HD int fd_exact(double x, double y, int ffd)
 {
     double fi = dm_atan2(x, y);
@@ -189,14 +205,24 @@ DSMC_HD int fd_exact(double x, double y, int ffd)
     return fd;
 }
 
-/* Longitude sector of (x, y): unchanged if safely inside sector `fd`. */
+/* True if (x, y) is farther than eps from both boundaries of sector k and inside it. */
+DSMC_HD bool fd_inside(double x, double y, int k, double eps, const double *cs)
+{
+    double dlo = x * cs[2 * k] - y * cs[2 * k + 1];
+    double dhi = x * cs[2 * k + 2] - y * cs[2 * k + 3];
+    return dlo > eps && dhi < -eps;
+}
+
+/* Longitude sector of (x, y): the current sector or a neighbour when the point is
+ * safely inside it (a sign test against the boundary directions), otherwise the
+ * exact ceil(atan2(x, y) / 2pi * ffd) - 1. */
 DSMC_HD int fd_update(double x, double y, int fd, int ffd, const double *cs)
 {
-    double r = fabs(x) + fabs(y);
-    double eps = DSMC_FD_MARGIN * r;
-    double dlo = x * cs[2 * fd] - y * cs[2 * fd + 1];
-    double dhi = x * cs[2 * fd + 2] - y * cs[2 * fd + 3];
-    if (dlo > eps && dhi < -eps) return fd;
+    double eps = DSMC_FD_MARGIN * (fabs(x) + fabs(y));
+    if (fd_inside(x, y, fd, eps, cs)) return fd;
+    int up = fd + 1 == ffd ? 0 : fd + 1, dn = fd == 0 ? ffd - 1 : fd - 1;
+    if (fd_inside(x, y, up, eps, cs)) return up;
+    if (fd_inside(x, y, dn, eps, cs)) return dn;
     return fd_exact(x, y, ffd);
 }
 
@@ -224,15 +250,7 @@ DSMC_HD int table_walk(const double *tab, int k, int kmax, double v)
 template <class Ops>
 DSMC_HD void lane_flush(DsmcLane &L, const DsmcEnv &E)
 {
-    if (L.acc_cell >= 0) {
-        Ops::add_f64(&E.density[L.acc_cell], L.acc_w);
-        if (E.vmom) {
-            Ops::add_f64(&E.vmom[(size_t)L.acc_cell * 3 + 0], L.acx);
-            Ops::add_f64(&E.vmom[(size_t)L.acc_cell * 3 + 1], L.acy);
-            Ops::add_f64(&E.vmom[(size_t)L.acc_cell * 3 + 2], L.acz);
-        }
-        if (E.vcount) Ops::add_u64(&E.vcount[L.acc_cell], (unsigned long long)L.acc_n);
-    }
+    if (L.acc_cell >= 0) Ops::deposit(E, L.acc_cell, L.acc_w, L.acx, L.acy, L.acz, L.acc_n);
     L.acc_cell = -1;
 }
 

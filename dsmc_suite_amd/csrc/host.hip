@@ -128,9 +128,12 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
         if ((rc = rd->vmom.zeros(cells * 3 * sizeof(double), st))) return rc;
         E.vmom = (double *)rd->vmom.ptr;
     }
-    if (r->vcount) {
+    if (r->vcount || p->mode == DSMC_MODE_PHYSICAL) {
+        /* physical particles keep W = 1, so the density is the deposit count: one
+         * atomic per cell visit instead of two */
         if ((rc = rd->vcount.zeros(cells * sizeof(unsigned long long), st))) return rc;
         E.vcount = (unsigned long long *)rd->vcount.ptr;
+        E.density_from_count = p->mode == DSMC_MODE_PHYSICAL;
     }
     int rows = advanced ? 9 : 5;
     E.ttime_rows = rows;
@@ -281,6 +284,11 @@ int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st
     if (r->vmom && rd->vmom.ptr) HIP_OK(hipMemcpyAsync(r->vmom, rd->vmom.ptr, cells * 24, hipMemcpyDeviceToHost, st));
     if (r->vcount && rd->vcount.ptr)
         HIP_OK(hipMemcpyAsync(r->vcount, rd->vcount.ptr, cells * 8, hipMemcpyDeviceToHost, st));
+    std::vector<unsigned long long> cnt_tmp;
+    if (rd->E.density_from_count && r->density && !r->vcount) {
+        cnt_tmp.resize(cells);
+        HIP_OK(hipMemcpyAsync(cnt_tmp.data(), rd->vcount.ptr, cells * 8, hipMemcpyDeviceToHost, st));
+    }
     if (r->ttime) {
         if (rd->ttime.ptr)
             HIP_OK(hipMemcpyAsync(r->ttime, rd->ttime.ptr, rd->ttime.bytes, hipMemcpyDeviceToHost, st));
@@ -290,6 +298,10 @@ int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st
     if (r->fate) HIP_OK(hipMemcpyAsync(r->fate, rd->fate.ptr, 4 * n, hipMemcpyDeviceToHost, st));
     if (r->hops) HIP_OK(hipMemcpyAsync(r->hops, rd->hops.ptr, 4 * n, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
+    if (rd->E.density_from_count && r->density) {
+        const int64_t *c = r->vcount ? r->vcount : (const int64_t *)cnt_tmp.data();
+        for (size_t i = 0; i < cells; i++) r->density[i] = (double)c[i];
+    }
     r->vrec_n = 0;
     if (rd->E.vrec) {
         int64_t nrec = (int64_t)rtot;

@@ -28,12 +28,77 @@
 
 static_assert(sizeof(DsmcRec) == 256, "particle record must stay 256 bytes");
 
+/* Deposits straight into the device grid (global FP64 atomics). */
+__device__ static inline void grid_deposit(const DsmcEnv &E, int cell, double w, double vx, double vy,
+                                           double vz, int n)
+{
+    if (!E.density_from_count) atomicAdd(&E.density[cell], w);
+    if (E.vmom) {
+        atomicAdd(&E.vmom[(size_t)cell * 3 + 0], vx);
+        atomicAdd(&E.vmom[(size_t)cell * 3 + 1], vy);
+        atomicAdd(&E.vmom[(size_t)cell * 3 + 2], vz);
+    }
+    if (E.vcount) atomicAdd(&E.vcount[cell], (unsigned long long)n);
+}
+
 struct DsmcOpsDev {
+    __device__ static inline void deposit(const DsmcEnv &E, int cell, double w, double vx, double vy,
+                                          double vz, int n)
+    {
+        grid_deposit(E, cell, w, vx, vy, vz, n);
+    }
     __device__ static inline void add_f64(double *p, double v) { atomicAdd(p, v); }
     __device__ static inline void add_u64(unsigned long long *p, unsigned long long v) { atomicAdd(p, v); }
     __device__ static inline int32_t claim_i32(int32_t *p) { return atomicAdd(p, 1); }
     __device__ static inline unsigned long long claim_u64(unsigned long long *p) { return atomicAdd(p, 1ull); }
 };
+
+/* LDS-privatised deposit of the flight kernel: a per-block table of DEP_SLOTS grid
+ * cells.  A cell claims the slot its hash selects with one LDS compare-and-swap
+ * (slots are never evicted during the launch); deposits of claimed cells are LDS
+ * atomics, others fall through to the global grid.  At the end of the launch each
+ * block merges its claimed slots into the grid with one global atomic per value. */
+#define DEP_SLOTS 512
+__shared__ int32_t dep_tag[DEP_SLOTS];
+__shared__ int32_t dep_n[DEP_SLOTS];
+__shared__ double dep_w[DEP_SLOTS], dep_vx[DEP_SLOTS], dep_vy[DEP_SLOTS], dep_vz[DEP_SLOTS];
+
+__device__ static inline int dep_slot(int cell) { return (int)(((uint32_t)cell * 2654435761u) >> 23); }
+
+struct DsmcOpsLds {
+    __device__ static inline void deposit(const DsmcEnv &E, int cell, double w, double vx, double vy,
+                                          double vz, int n)
+    {
+        int s = dep_slot(cell);
+        int t = dep_tag[s];
+        if (t == -1) {
+            t = atomicCAS(&dep_tag[s], -1, cell);
+            if (t == -1) t = cell;
+        }
+        if (t != cell) { grid_deposit(E, cell, w, vx, vy, vz, n); return; }
+        atomicAdd(&dep_w[s], w);
+        if (E.vmom) { atomicAdd(&dep_vx[s], vx); atomicAdd(&dep_vy[s], vy); atomicAdd(&dep_vz[s], vz); }
+        if (E.vcount) atomicAdd(&dep_n[s], n);
+    }
+    __device__ static inline void add_f64(double *p, double v) { atomicAdd(p, v); }
+    __device__ static inline void add_u64(unsigned long long *p, unsigned long long v) { atomicAdd(p, v); }
+    __device__ static inline int32_t claim_i32(int32_t *p) { return atomicAdd(p, 1); }
+    __device__ static inline unsigned long long claim_u64(unsigned long long *p) { return atomicAdd(p, 1ull); }
+};
+
+__device__ static inline void dep_init()
+{
+    for (int i = threadIdx.x; i < DEP_SLOTS; i += blockDim.x) {
+        dep_tag[i] = -1; dep_n[i] = 0;
+        dep_w[i] = 0.0; dep_vx[i] = 0.0; dep_vy[i] = 0.0; dep_vz[i] = 0.0;
+    }
+}
+
+__device__ static inline void dep_merge(const DsmcEnv &E)
+{
+    for (int i = threadIdx.x; i < DEP_SLOTS; i += blockDim.x)
+        if (dep_tag[i] >= 0) grid_deposit(E, dep_tag[i], dep_w[i], dep_vx[i], dep_vy[i], dep_vz[i], dep_n[i]);
+}
 
 __device__ static inline uint64_t lanemask_lt()
 {
@@ -103,6 +168,8 @@ flight_kernel(DsmcEnv E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int p
         TB.tq = lds_tabs;
         TB.hr = lds_tabs + ntq;
         TB.cs = lds_tabs + ntq + nhr;
+        dep_init();
+        __syncthreads();
     }
     const unsigned int nq = Q.ctl[2 * p];           /* fly queue p is not appended to in this launch */
     unsigned int *head = &Q.ctl[2 * p + 1];
@@ -143,7 +210,8 @@ flight_kernel(DsmcEnv E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int p
         if (BATCH) { TB.tq = E.tq; TB.hr = E.hr; TB.cs = E.cs; }
         for (;;) {
             if (L.phase == LANE_FLIGHT) {
-                lane_step<DsmcOpsDev, MODE, ADV>(L, E, TB);
+                if (BATCH) lane_step<DsmcOpsDev, MODE, ADV>(L, E, TB);
+                else lane_step<DsmcOpsLds, MODE, ADV>(L, E, TB);
                 if (L.phase == LANE_FLIGHT && L.steps >= budget_end) L.phase = LANE_IDLE + 8;
             }
             uint64_t ended = __ballot(L.phase != LANE_FLIGHT && L.phase != LANE_IDLE);
@@ -153,13 +221,18 @@ flight_kernel(DsmcEnv E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int p
         bool hop_end = L.phase == LANE_LANDED || L.phase == LANE_ENDED;
         bool paused = L.phase == LANE_IDLE + 8;
         if (hop_end || paused) {
-            lane_flush<DsmcOpsDev>(L, E);
+            if (BATCH) lane_flush<DsmcOpsDev>(L, E);
+            else lane_flush<DsmcOpsLds>(L, E);
             if (paused) L.phase = LANE_FLIGHT;
             rec_store_flight<MODE, ADV>(L, rec[slot]);
         }
         wave_append(hop_end, slot, Q.svc, &Q.ctl[4]);
         wave_append(paused, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
         if (hop_end || paused) L.phase = LANE_IDLE;
+    }
+    if (!BATCH) {
+        __syncthreads();
+        dep_merge(E0);
     }
 }
 
