@@ -70,44 +70,20 @@ def cpu_baseline(p, surf, sample: int, threads: int) -> dict:
                       f"{dt:.2f} s, oracle/libcpu_ref.so, OpenMP schedule(dynamic,100))"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--particles", type=int, default=1_000_000)
-    ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--blocks-per-cu", type=int, default=0)
-    ap.add_argument("--relaunch-batch", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=60000)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-moments", action="store_true", help="density only (no velocity moments)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
-        sys.exit(2)
-
-    lib = capi.load()
-    p = capi.preset(lib, PRESET_CERES_PHYSICAL, 36, 72, n=args.particles, seed=args.seed)
-    surf = icosphere(lib, 3)
+def plain_run(lib, surf, n, args, shape) -> dict:
+    """Stages one plain run of n particles and times args.steps complete executions."""
+    p = capi.preset(lib, PRESET_CERES_PHYSICAL, 36, 72, n=n, seed=args.seed)
     outs = Buffers(p, 5, per_particle=False, moments=not args.no_moments)
     st = C.c_int(0)
     run = lib.dsmc_run_create(C.byref(p), C.byref(surf.surface), C.byref(outs.result), C.byref(st))
     if not run:
         raise RuntimeError(f"dsmc_run_create failed: {st.value}")
-    shape = None
-    if args.block or args.blocks_per_cu or args.relaunch_batch:
-        shape = capi.LaunchShape(args.block or 256, args.blocks_per_cu or 2, args.relaunch_batch or 16)
     shape_p = C.byref(shape) if shape is not None else None
-
     for _ in range(args.warmup):
         if lib.dsmc_run_launch(run, shape_p) or lib.dsmc_run_sync(run):
             raise RuntimeError("warmup launch failed")
     kms, fms = [], []
-    rounds = C.c_int(0)
-    fl = C.c_double(0)
+    rounds, fl = C.c_int(0), C.c_double(0)
     lib.dsmc_run_sync(run)
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -122,26 +98,64 @@ def main():
         raise RuntimeError("collect failed")
     lib.dsmc_run_destroy(run)
     steps_per_run = outs.result.total_steps
-    value = steps_per_run * args.steps / wall
     favg = float(np.mean(fms)) / 1e3          # flight-kernel device time per run
     achieved = FLOPS_PER_STEP * steps_per_run / favg / 1e12
+    return {"params": p, "value": steps_per_run * args.steps / wall, "ms_per_run": wall / args.steps * 1e3,
+            "particles": n, "particle_steps_per_run": steps_per_run,
+            "fates": {k: int(getattr(outs.result, k)) for k in
+                      ("n_escaped", "n_frozen", "n_photolysed", "n_capped")},
+            "run_device_ms": [round(x, 3) for x in kms],
+            "flight_kernel_ms": [round(x, 3) for x in fms], "rounds": rounds.value,
+            "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                         "flops_per_particle_step": FLOPS_PER_STEP,
+                         "kernel": "flight_kernel<1,0,false>", "kernel_avg_ms_per_run": favg * 1e3}}
 
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--particles", type=int, default=10_000_000)
+    ap.add_argument("--also-particles", type=int, default=1_000_000, help="second plain size (0: none)")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--relaunch-batch", type=int, default=0)
+    ap.add_argument("--cpu-sample", type=int, default=60000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-moments", action="store_true", help="density only (no velocity moments)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        sys.exit(2)
+
+    lib = capi.load()
+    surf = icosphere(lib, 3)
+    shape = None
+    if args.block or args.blocks_per_cu or args.relaunch_batch:
+        shape = capi.LaunchShape(args.block or 256, args.blocks_per_cu or 4, args.relaunch_batch or 16)
+    main_run = plain_run(lib, surf, args.particles, args, shape)
+    p = main_run.pop("params")
     line = {
-        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "metric": METRIC, "value": main_run["value"], "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": main_run["ms_per_run"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded icosphere surface and Philox draws)",
-        "config": {"workload": "plain single run: Ceres H2O ballistic exosphere, physical mode, "
-                               "icosphere 642/1280, dt 10 s, <=20000 steps, grid 50x36x72 exp to 20R",
-                   "particles": args.particles, "particle_steps_per_run": steps_per_run,
-                   "fates": {k: int(getattr(outs.result, k)) for k in
-                             ("n_escaped", "n_frozen", "n_photolysed", "n_capped")},
-                   "run_device_ms": [round(x, 3) for x in kms],
-                   "flight_kernel_ms": [round(x, 3) for x in fms], "rounds": rounds.value},
-        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "flops_per_particle_step": FLOPS_PER_STEP},
+        "config": {"workload": f"plain single run, Ceres H2O ballistic exosphere (physical mode) at "
+                               f"{args.particles:.0e} particles: icosphere 642/1280, dt 10 s, <=20000 steps, "
+                               f"grid 50x36x72 exponential to 20R, density + velocity moments per cell",
+                   **{k: v for k, v in main_run.items() if k not in ("value", "ms_per_run", "roofline")}},
+        "roofline": main_run["roofline"],
     }
+    if args.also_particles:
+        other = plain_run(lib, surf, args.also_particles, args, shape)
+        other.pop("params")
+        line["config"][f"plain_{args.also_particles:.0e}"] = {
+            "value": other["value"], "ms_per_run": other["ms_per_run"],
+            "flight_kernel_ms": other["flight_kernel_ms"], "rounds": other["rounds"]}
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(p, surf, args.cpu_sample, threads)
