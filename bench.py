@@ -30,6 +30,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 from dsmc_suite_amd import capi  # noqa: E402
+from dsmc_suite_amd import workloads as W  # noqa: E402
 from dsmc_suite_amd.capi import PRESET_CERES_PHYSICAL, Buffers, SurfaceArrays, ptr  # noqa: E402
 
 METRIC = "particle-steps/sec FP64, plain vs batched/served, 1 MI355X; p50/p99 run latency"
@@ -109,7 +110,129 @@ def plain_run(lib, surf, n, args, shape) -> dict:
             "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                          "flops_per_particle_step": FLOPS_PER_STEP,
-                         "kernel": "flight_kernel<1,0,false>", "kernel_avg_ms_per_run": favg * 1e3}}
+                         "kernel": "phys_kernel<false>" if (shape is None or shape.service_inline) else
+                         "flight_kernel<1,0>", "kernel_avg_ms_per_run": favg * 1e3}}
+
+
+def percentiles(x) -> dict:
+    x = np.asarray(x, dtype=np.float64)
+    if x.size == 0:
+        return {"p50_ms": None, "p99_ms": None, "max_ms": None}
+    return {"p50_ms": float(np.percentile(x, 50)), "p99_ms": float(np.percentile(x, 99)),
+            "max_ms": float(x.max())}
+
+
+def batched_run(lib, args, shape) -> dict:
+    """The batched workload: 64 independent runs (counts log-uniform on [1e4, 2e6],
+    seed 1; body and subsolar longitude drawn per run), icosphere 2562/5120,
+    grid 40x36x72, in one pool; plus the same runs one after another through the
+    plain single-run path."""
+    specs = W.batch_specs(args.batch_runs, seed=1, scale=args.batch_scale)
+    params, surfs = W.build_runs(lib, specs, levels=4, hd=40)
+    pa, sa = W.as_arrays(params, surfs)
+    n = len(specs)
+    st = C.c_int(0)
+    t_tune = time.perf_counter()
+    b = lib.dsmc_batch_create(n, pa, sa, 0, C.byref(shape) if shape is not None else None, C.byref(st))
+    if not b:
+        raise RuntimeError(f"dsmc_batch_create failed: {st.value}")
+    t_tune = time.perf_counter() - t_tune
+    bufs = [Buffers(p, 5, per_particle=False, moments=True) for p in params]
+    arr = (C.POINTER(capi.Result) * n)(*[C.pointer(x.result) for x in bufs])
+    for _ in range(args.warmup):
+        if lib.dsmc_batch_launch(b, arr):
+            raise RuntimeError("batch warmup failed")
+    walls, done = [], np.zeros(n)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        if lib.dsmc_batch_launch(b, arr):
+            raise RuntimeError("batch launch failed")
+    wall = time.perf_counter() - t0
+    w, rounds, used = C.c_double(), C.c_int64(), capi.LaunchShape()
+    lib.dsmc_batch_timing(b, C.byref(w), capi.ptr(done, C.c_double), C.byref(rounds), C.byref(used))
+    lib.dsmc_batch_destroy(b)
+    steps = sum(int(x.result.total_steps) for x in bufs)
+    out = {"value": steps * args.steps / wall, "unit": "particle-steps/s", "runs": n,
+           "particles": int(sum(s.n for s in specs)), "particle_steps_per_batch": steps,
+           "ms_per_batch": wall / args.steps * 1e3, "rounds": int(rounds.value),
+           "run_finish_ms": percentiles(done), "autotune_s": round(t_tune, 3),
+           "shape": {"block": used.block, "blocks_per_cu": used.blocks_per_cu,
+                     "relaunch_batch": used.relaunch_batch, "step_budget": used.step_budget,
+                     "service_inline": used.service_inline},
+           "graph": "hipGraph of 8 rounds (flight+service+reset) replayed"}
+    if args.sequential:
+        t_seq, seq_steps = 0.0, 0
+        for p, s in zip(params, surfs):
+            ob = Buffers(p, 5, per_particle=False, moments=True)
+            run = lib.dsmc_run_create(C.byref(p), C.byref(s.surface), C.byref(ob.result), C.byref(st))
+            if not run:
+                raise RuntimeError(f"dsmc_run_create failed: {st.value}")
+            t1 = time.perf_counter()
+            if lib.dsmc_run_launch(run, None) or lib.dsmc_run_sync(run):
+                raise RuntimeError("plain launch failed")
+            t_seq += time.perf_counter() - t1
+            lib.dsmc_run_collect(run, C.byref(ob.result))
+            seq_steps += int(ob.result.total_steps)
+            lib.dsmc_run_destroy(run)
+        out["plain_sequential"] = {"value": seq_steps / t_seq, "ms": t_seq * 1e3, "particle_steps": seq_steps}
+        out["speedup_vs_plain_sequential"] = out["value"] / (seq_steps / t_seq)
+    return out
+
+
+def served_run(lib, args, shape) -> dict:
+    """Poisson request stream (20 req/s) of ballistic runs, counts log-uniform on
+    [1e4, 4e6], body and subsolar longitude drawn per request, through the
+    continuous-batching server; latency = arrival to result ready."""
+    stream = W.request_stream(args.serve_rate, args.serve_seconds, seed=5, scale=args.serve_scale)
+    skipped = sum(1 for _, s in stream if s.collisional)
+    stream = [(t, s) for t, s in stream if not s.collisional]
+    specs = [s for _, s in stream]
+    params, surfs = W.build_runs(lib, specs, levels=4, hd=40)
+    sv = lib.dsmc_server_create(args.serve_budget_gb << 30, 256, 0)
+    if not sv:
+        raise RuntimeError("dsmc_server_create failed")
+    bufs = [Buffers(p, 5, per_particle=False, moments=True) for p in params]
+    tickets, t_sub = [0] * len(stream), [0.0] * len(stream)
+    t0 = time.perf_counter()
+    i, outstanding = 0, set()
+    first = True
+    while i < len(stream) or outstanding:
+        now = time.perf_counter() - t0
+        while i < len(stream) and stream[i][0] <= now:
+            tk = C.c_uint64()
+            rc = lib.dsmc_server_submit(sv, C.byref(params[i]), C.byref(surfs[i].surface),
+                                        C.byref(bufs[i].result), C.byref(tk))
+            if rc:
+                raise RuntimeError(f"submit failed: {rc}")
+            if first:
+                if shape is not None and lib.dsmc_server_set_shape(sv, C.byref(shape)):
+                    raise RuntimeError("set_shape failed")
+                first = False
+            tickets[i], t_sub[i] = tk.value, time.perf_counter() - t0
+            outstanding.add(i)
+            i += 1
+        if outstanding:
+            if lib.dsmc_server_step(sv) < 0:
+                raise RuntimeError("server step failed")
+            for j in list(outstanding):
+                st = lib.dsmc_server_poll(sv, tickets[j])
+                if st < 0:
+                    raise RuntimeError(f"request {j} failed: {st}")
+                if st == 1:
+                    outstanding.discard(j)
+        elif i < len(stream):
+            time.sleep(max(0.0, stream[i][0] - (time.perf_counter() - t0)))
+    t_end = time.perf_counter() - t0
+    stats = capi.ServerStats()
+    lib.dsmc_server_get_stats(sv, C.byref(stats))
+    lib.dsmc_server_destroy(sv)
+    lat = [bufs[j].result.elapsed_ms + (t_sub[j] - stream[j][0]) * 1e3 for j in range(len(stream))]
+    steps = sum(int(b.result.total_steps) for b in bufs)
+    return {"value": steps / t_end, "unit": "particle-steps/s", "requests": len(stream),
+            "collisional_requests_not_served": skipped, "offered_seconds": args.serve_seconds,
+            "rate_per_s": args.serve_rate, "wall_s": t_end, "particles": int(sum(s.n for s in specs)),
+            "particle_steps": steps, "latency": percentiles(lat), "rounds": int(stats.rounds),
+            "slots": int(stats.slots)}
 
 
 def main():
@@ -123,9 +246,20 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--relaunch-batch", type=int, default=0)
+    ap.add_argument("--step-budget", type=int, default=0)
+    ap.add_argument("--service-inline", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=60000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-moments", action="store_true", help="density only (no velocity moments)")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-runs", type=int, default=64)
+    ap.add_argument("--batch-scale", type=float, default=1.0, help="multiplies every run's particle count")
+    ap.add_argument("--sequential", type=int, default=1, help="also time the batch's runs one by one")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-scale", type=float, default=1.0)
+    ap.add_argument("--serve-budget-gb", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -135,8 +269,9 @@ def main():
     lib = capi.load()
     surf = icosphere(lib, 3)
     shape = None
-    if args.block or args.blocks_per_cu or args.relaunch_batch:
-        shape = capi.LaunchShape(args.block or 256, args.blocks_per_cu or 4, args.relaunch_batch or 16)
+    if args.block or args.blocks_per_cu or args.relaunch_batch or args.step_budget or args.service_inline != 1:
+        shape = capi.LaunchShape(args.block or 256, args.blocks_per_cu or 4, args.relaunch_batch or 16,
+                                 args.step_budget or 1024, args.service_inline)
     main_run = plain_run(lib, surf, args.particles, args, shape)
     p = main_run.pop("params")
     line = {
@@ -156,6 +291,16 @@ def main():
         line["config"][f"plain_{args.also_particles:.0e}"] = {
             "value": other["value"], "ms_per_run": other["ms_per_run"],
             "flight_kernel_ms": other["flight_kernel_ms"], "rounds": other["rounds"]}
+    tuned = None
+    if not args.no_batch:
+        bt = batched_run(lib, args, shape)
+        line["batched"] = bt
+        s = bt["shape"]
+        tuned = capi.LaunchShape(s["block"], s["blocks_per_cu"], s["relaunch_batch"], s["step_budget"],
+                                 s["service_inline"])
+        line["batched_vs_plain"] = bt["value"] / main_run["value"]
+    if not args.no_serve:
+        line["served"] = served_run(lib, args, shape if shape is not None else tuned)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(p, surf, args.cpu_sample, threads)

@@ -78,7 +78,14 @@ class Phase2Input(C.Structure):
 
 
 class LaunchShape(C.Structure):
-    _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("relaunch_batch", C.c_int)]
+    _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("relaunch_batch", C.c_int),
+                ("step_budget", C.c_int), ("service_inline", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_int64), ("particle_steps", C.c_int64), ("queued", C.c_int64),
+                ("running", C.c_int64), ("particles_in_flight", C.c_int64), ("slots", C.c_int64),
+                ("rounds", C.c_int64), ("shape", LaunchShape)]
 
 
 def ptr(a: np.ndarray | None, ctype):
@@ -173,13 +180,22 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.dsmc_icosphere_build.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,
                                          C.c_double, C.c_double, _dp, _i32p, _dp, _dp]
     lib.dsmc_icosphere_rasterise.argtypes = [S, C.c_int, C.c_int, _dp]
-    lib.dsmc_server_create.argtypes = [C.c_size_t, C.c_int]
+    lib.dsmc_batch_create.argtypes = [C.c_int, P, S, C.c_int, C.POINTER(LaunchShape), C.POINTER(C.c_int)]
+    lib.dsmc_batch_create.restype = C.c_void_p
+    lib.dsmc_batch_launch.argtypes = [C.c_void_p, C.POINTER(R)]
+    lib.dsmc_batch_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), _dp, _i64p, C.POINTER(LaunchShape)]
+    lib.dsmc_batch_destroy.argtypes = [C.c_void_p]
+    lib.dsmc_batch_destroy.restype = None
+    lib.dsmc_server_create.argtypes = [C.c_size_t, C.c_int, C.c_int]
     lib.dsmc_server_create.restype = C.c_void_p
+    lib.dsmc_server_set_shape.argtypes = [C.c_void_p, C.POINTER(LaunchShape)]
     lib.dsmc_server_submit.argtypes = [C.c_void_p, P, S, R, C.POINTER(C.c_uint64)]
     lib.dsmc_server_poll.argtypes = [C.c_void_p, C.c_uint64]
     lib.dsmc_server_wait.argtypes = [C.c_void_p, C.c_uint64]
     lib.dsmc_server_cancel.argtypes = [C.c_void_p, C.c_uint64]
+    lib.dsmc_server_release.argtypes = [C.c_void_p, C.c_uint64]
     lib.dsmc_server_step.argtypes = [C.c_void_p]
+    lib.dsmc_server_get_stats.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
     lib.dsmc_server_destroy.argtypes = [C.c_void_p]
     lib.dsmc_server_destroy.restype = None
     lib.dsmc_run_create.argtypes = [P, S, R, C.POINTER(C.c_int)]

@@ -28,7 +28,7 @@ struct DevBuf {
 struct RunDev {
     DsmcEnv E;
     HostTables T;
-    DevBuf tq, hr, cs, vt_speeds, vt_count, Tsurf, cdf, enorm, vert, faces, face_T;
+    DevBuf tq, hr, cs, vt_speeds, vt_count, Tsurf, cdf, enorm, fnorm, vert, faces, face_T;
     DevBuf density, horizon, vmom, vcount, ttime, end_state, steps, fate, hops;
     DevBuf vrec, vrec_i, count_nn, ctl;
     /* advanced */
@@ -48,6 +48,7 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
 int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int timed = 0);
 int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st);
 dsmc_launch_shape default_shape(const DsmcEnv &E);
+bool shape_ok(const dsmc_launch_shape &s);
 
 }  // namespace dsmc
 

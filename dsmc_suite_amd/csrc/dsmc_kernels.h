@@ -14,16 +14,38 @@ struct DsmcQueues {
     unsigned int *ctl;
 };
 
+/* A pool of runs sharing one set of particle slots and queues (multi-run
+ * launches).  Run g's particles carry rec.run = g; envs[g] / steps[g] describe
+ * it.  Every run of a pool has the pool's mode and grid dimensions. */
+struct DsmcPoolDev {
+    DsmcRec *rec;
+    const DsmcEnv *envs;       /* [max_runs] full environments (start/service) */
+    const DsmcStepEnv *steps;  /* [max_runs] what a flight step reads */
+    const double *tq, *cs;     /* latitude thresholds, sector directions (pool dims) */
+    const double *hr;          /* radial threshold tables, [nhr] doubles in all */
+    int ftd, ffd, hd, nhr;
+    int mode, adv;
+};
+
 extern "C" {
-hipError_t dsmc_launch_start(const DsmcEnv *E0, const DsmcEnv *envs, const int64_t *run_base, int nruns,
-                             DsmcRec *rec, int64_t slot0, int64_t nslots, DsmcQueues Q, int p, int grid,
-                             hipStream_t st);
-/* flight kernel of round p */
-hipError_t dsmc_launch_flight(const DsmcEnv *E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int p,
-                              int grid, int refill_batch, int64_t step_budget, hipStream_t st);
-/* service kernel of round p, then the queue reset */
-hipError_t dsmc_launch_service(const DsmcEnv *E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int p,
-                               int grid, hipStream_t st);
+/* Starts particles [0, n) of one run in slots slot0.. (records carry run_id). */
+hipError_t dsmc_launch_start(const DsmcEnv *E, int32_t run_id, DsmcRec *rec, int64_t slot0, int64_t n,
+                             DsmcQueues Q, int p, int grid, hipStream_t st);
+/* flight kernel of round p (one run) */
+hipError_t dsmc_launch_flight(const DsmcEnv *E, DsmcRec *rec, DsmcQueues Q, int p, int grid, int block,
+                              int refill_batch, int64_t step_budget, hipStream_t st);
+/* service kernel of round p (one run), then the queue reset */
+hipError_t dsmc_launch_service(const DsmcEnv *E, DsmcRec *rec, DsmcQueues Q, int p, int grid, hipStream_t st);
+/* the same two for a pool of runs */
+hipError_t dsmc_launch_pool_flight(const DsmcPoolDev *P, DsmcQueues Q, int p, int grid, int block,
+                                   int refill_batch, int64_t step_budget, hipStream_t st);
+hipError_t dsmc_launch_pool_service(const DsmcPoolDev *P, DsmcQueues Q, int p, int grid, hipStream_t st);
+/* physical runs: flight with the landing service inline, then the queue reset;
+ * one run (E, P == NULL) or a pool (P) */
+hipError_t dsmc_launch_phys(const DsmcEnv *E, const DsmcPoolDev *P, DsmcRec *rec, DsmcQueues Q, int p, int grid,
+                            int block, int svc_batch, int64_t step_budget, hipStream_t st);
+/* n zeros of 8 bytes (graph-safe clear) */
+hipError_t dsmc_launch_fill0(void *p, size_t n8, hipStream_t st);
 hipError_t dsmc_launch_verlet(double *state, int n, int nsteps, double GM, double R,
                               double max_height, double h, double *traj, hipStream_t s);
 hipError_t dsmc_launch_math(int fn, const double *x, const double *y, double *out, int n,

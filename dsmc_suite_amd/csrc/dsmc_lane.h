@@ -71,6 +71,7 @@ struct DsmcEnv {
     const double *vert;       /* [nvert*3] */
     const int32_t *faces;     /* all levels, [sum*3] */
     const double *enorm;      /* all levels, [sum*9]: cross(a,b), cross(b,c), cross(c,a) */
+    const double *fnorm;      /* finest level, [nfine*9]: the same normals, unit length */
     int level_off[12];
     /* advanced inputs */
     const double *copy_atm;   /* scaled phase-1 density, [atm_total] */
@@ -100,6 +101,19 @@ struct DsmcEnv {
     unsigned long long *counters;
 };
 
+/* What one physical flight step reads, held per lane in registers when the lanes
+ * of a wavefront belong to different runs (multi-run pools).  Field names match
+ * DsmcEnv so the step is one template for both.  ftd/ffd/hd are the pool's. */
+struct DsmcStepEnv {
+    double h, GM, c1, c2, R, max_height, sun_x, sun_y;
+    int64_t max_steps;
+    int ftd, ffd, hd, density_from_count;
+    double *density, *vmom;
+    unsigned long long *vcount;
+    int hr_off;               /* this run's radial thresholds within the pool's table */
+    int pad_;
+};
+
 struct DsmcLane {
     double sx, sy, sz, vx, vy, vz, ox, oy, oz;
     double Ri, W, time, flight_time, thi, fi;
@@ -110,6 +124,7 @@ struct DsmcLane {
     uint64_t k, blk;
     uint32_t rb0, rb1, rb2, rb3;
     int phase, fate, hops, num_col, td, fd, ht, acc_cell, acc_n, rec_key, rec_full, rotated;
+    int face;                        /* physical: face of the last launch */
 };
 
 /* A particle between kernels: one 256-byte record per slot in device memory. */
@@ -129,7 +144,8 @@ struct DsmcCellTabs {
 
 /* Atomic/plain accumulation, supplied by the caller (GPU atomics or host adds). */
 struct DsmcOpsHost {
-    static inline void deposit(const DsmcEnv &E, int cell, double w, double vx, double vy, double vz, int n)
+    template <class EnvT>
+    static inline void deposit(const EnvT &E, int cell, double w, double vx, double vy, double vz, int n)
     {
         if (!E.density_from_count) E.density[cell] += w;
         if (E.vmom) {
@@ -146,11 +162,21 @@ struct DsmcOpsHost {
 };
 
 /* ------------------------------------------------------------------ RNG */
+/* One Philox block; out of line on the device so that the several draws of an
+ * emission share one copy of the rounds (and its registers). */
+#if defined(__HIP_DEVICE_COMPILE__)
+__attribute__((noinline))
+#endif
+DSMC_HD dsmc_u32x4 lane_block(uint64_t seed, uint64_t pid, uint32_t stream, uint64_t b)
+{
+    return dsmc_rng_block(seed, pid, stream, b);
+}
+
 DSMC_HD uint32_t lane_word(DsmcLane &L, const DsmcEnv &E)
 {
     uint64_t b = L.k >> 2;
     if (b != L.blk) {
-        dsmc_u32x4 r = dsmc_rng_block(E.seed, (uint64_t)L.pid, E.stream, b);
+        dsmc_u32x4 r = lane_block(E.seed, (uint64_t)L.pid, E.stream, b);
         L.rb0 = r.w[0]; L.rb1 = r.w[1]; L.rb2 = r.w[2]; L.rb3 = r.w[3];
         L.blk = b;
     }
@@ -247,15 +273,15 @@ DSMC_HD int table_walk(const double *tab, int k, int kmax, double v)
 }
 
 /* ------------------------------------------------------------------ deposit */
-template <class Ops>
-DSMC_HD void lane_flush(DsmcLane &L, const DsmcEnv &E)
+template <class Ops, class EnvT>
+DSMC_HD void lane_flush(DsmcLane &L, const EnvT &E)
 {
     if (L.acc_cell >= 0) Ops::deposit(E, L.acc_cell, L.acc_w, L.acx, L.acy, L.acz, L.acc_n);
     L.acc_cell = -1;
 }
 
-template <class Ops>
-DSMC_HD void lane_deposit(DsmcLane &L, const DsmcEnv &E, int cell)
+template <class Ops, class EnvT>
+DSMC_HD void lane_deposit(DsmcLane &L, const EnvT &E, int cell)
 {
     if (cell != L.acc_cell) {
         lane_flush<Ops>(L, E);
@@ -289,13 +315,14 @@ DSMC_HD void lane_record(DsmcLane &L, const DsmcEnv &E)
 
 /* ------------------------------------------------------------------ end of a particle */
 struct DsmcCounts {
-    unsigned long long esc, frozen, photo, depleted, capped, coll, steps;
+    unsigned long long esc, frozen, photo, depleted, capped, coll, steps, done;
 };
 
+template <int MODE = 0>
 DSMC_HD void lane_finish(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
 {
     int64_t n = L.pid;
-    if (E.ttime) {
+    if (MODE == 0 && E.ttime) {
         int64_t N = E.nparticles;
         if (L.fate != 1 /* escaped: rows 1+ stay 0 */) {
             E.ttime[1 * N + n] = L.thi;
@@ -314,7 +341,7 @@ DSMC_HD void lane_finish(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
     if (E.end_state) {
         double *e = E.end_state + (size_t)n * 6;
         if (L.steps == 0) { e[0] = e[1] = e[2] = e[3] = e[4] = e[5] = 0.0; }
-        else if (L.rotated) { e[0] = L.lsx; e[1] = L.lsy; e[2] = L.lsz; e[3] = L.vx; e[4] = L.vy; e[5] = L.vz; }
+        else if (MODE == 0 && L.rotated) { e[0] = L.lsx; e[1] = L.lsy; e[2] = L.lsz; e[3] = L.vx; e[4] = L.vy; e[5] = L.vz; }
         else { e[0] = L.sx; e[1] = L.sy; e[2] = L.sz; e[3] = L.vx; e[4] = L.vy; e[5] = L.vz; }
     }
     cnt.steps += (unsigned long long)L.steps;
@@ -327,6 +354,7 @@ DSMC_HD void lane_finish(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
     case 5: cnt.capped++; break;
     default: break;
     }
+    cnt.done++;
     L.phase = LANE_DONE;
 }
 
@@ -335,10 +363,11 @@ template <class Ops>
 DSMC_HD void counts_flush(DsmcCounts &cnt, const DsmcEnv &E)
 {
     if (!E.counters) return;
-    unsigned long long v[7] = {cnt.esc, cnt.frozen, cnt.photo, cnt.depleted, cnt.capped, cnt.coll, cnt.steps};
-    for (int i = 0; i < 7; i++)
+    unsigned long long v[8] = {cnt.esc, cnt.frozen, cnt.photo, cnt.depleted, cnt.capped, cnt.coll, cnt.steps,
+                               cnt.done};
+    for (int i = 0; i < 8; i++)
         if (v[i]) Ops::add_u64(&E.counters[i], v[i]);
-    cnt.esc = cnt.frozen = cnt.photo = cnt.depleted = cnt.capped = cnt.coll = cnt.steps = 0;
+    cnt.esc = cnt.frozen = cnt.photo = cnt.depleted = cnt.capped = cnt.coll = cnt.steps = cnt.done = 0;
 }
 
 /* ------------------------------------------------------------------ records */
@@ -375,6 +404,9 @@ DSMC_HD void rec_store_flight(const DsmcLane &L, DsmcRec &R)
     if (ADV) { R.real_col_p = L.real_col_p; R.num_col = L.num_col; R.k = L.k; }
 }
 
+/* The whole particle (start and service kernels).  The record's ii slot holds
+ * the hop step counter in compat mode and the launch face in physical mode. */
+template <int MODE>
 DSMC_HD void rec_load_all(DsmcLane &L, const DsmcRec &R)
 {
     L.sx = R.sx; L.sy = R.sy; L.sz = R.sz; L.vx = R.vx; L.vy = R.vy; L.vz = R.vz;
@@ -383,11 +415,14 @@ DSMC_HD void rec_load_all(DsmcLane &L, const DsmcRec &R)
     L.rest_time = R.rest_time; L.real_col_p = R.real_col_p; L.col_p = R.col_p; L.col_lo = R.col_lo;
     L.lsx = R.lsx; L.lsy = R.lsy; L.lsz = R.lsz;
     L.steps = R.steps; L.n_day = R.n_day; L.n_photo = R.n_photo; L.pid = R.pid; L.k = R.k;
-    L.ii = R.ii; L.phase = R.phase; L.fate = R.fate; L.hops = R.hops; L.num_col = R.num_col;
+    if (MODE == 0) { L.ii = R.ii; L.face = -1; }
+    else { L.ii = 0; L.face = R.ii; }
+    L.phase = R.phase; L.fate = R.fate; L.hops = R.hops; L.num_col = R.num_col;
     L.td = R.td; L.fd = R.fd; L.ht = R.ht; L.rotated = R.rotated;
     L.blk = ~0ull; L.acc_cell = -1; L.rec_key = -1; L.rec_full = 0;
 }
 
+template <int MODE>
 DSMC_HD void rec_store_all(const DsmcLane &L, DsmcRec &R, int32_t run)
 {
     R.sx = L.sx; R.sy = L.sy; R.sz = L.sz; R.vx = L.vx; R.vy = L.vy; R.vz = L.vz;
@@ -396,12 +431,42 @@ DSMC_HD void rec_store_all(const DsmcLane &L, DsmcRec &R, int32_t run)
     R.rest_time = L.rest_time; R.real_col_p = L.real_col_p; R.col_p = L.col_p; R.col_lo = L.col_lo;
     R.lsx = L.lsx; R.lsy = L.lsy; R.lsz = L.lsz;
     R.steps = L.steps; R.n_day = L.n_day; R.n_photo = L.n_photo; R.pid = L.pid; R.k = L.k;
-    R.ii = (int32_t)L.ii; R.phase = L.phase; R.fate = L.fate; R.hops = L.hops; R.num_col = L.num_col;
+    R.ii = MODE == 0 ? (int32_t)L.ii : L.face;
+    R.phase = L.phase; R.fate = L.fate; R.hops = L.hops; R.num_col = L.num_col;
     R.td = L.td; R.fd = L.fd; R.ht = L.ht; R.rotated = L.rotated; R.run = run;
 }
 
+/* What a physical particle carries between launches of the flight kernel with
+ * the landing service inline (records written by rec_store_all load the same). */
+DSMC_HD void rec_load_phys(DsmcLane &L, const DsmcRec &R)
+{
+    L.sx = R.sx; L.sy = R.sy; L.sz = R.sz; L.vx = R.vx; L.vy = R.vy; L.vz = R.vz;
+    L.ox = R.ox; L.oy = R.oy; L.oz = R.oz;
+    L.Ri = R.Ri; L.W = R.W;
+    L.steps = R.steps; L.n_day = R.n_day; L.n_photo = R.n_photo; L.pid = R.pid; L.k = R.k;
+    L.phase = R.phase; L.fate = R.fate; L.hops = R.hops;
+    L.td = R.td; L.fd = R.fd; L.ht = R.ht;
+    L.face = R.ii;                   /* physical records keep the launch face in ii */
+    L.num_col = 0; L.rotated = 0;
+    L.blk = ~0ull; L.acc_cell = -1; L.rec_key = -1; L.rec_full = 0;
+}
+
+DSMC_HD void rec_store_phys(const DsmcLane &L, DsmcRec &R)
+{
+    R.sx = L.sx; R.sy = L.sy; R.sz = L.sz; R.vx = L.vx; R.vy = L.vy; R.vz = L.vz;
+    R.ox = L.ox; R.oy = L.oy; R.oz = L.oz;
+    R.Ri = L.Ri; R.W = L.W;
+    R.steps = L.steps; R.n_day = L.n_day; R.k = L.k;
+    R.phase = L.phase; R.fate = L.fate; R.hops = L.hops;
+    R.td = L.td; R.fd = L.fd; R.ht = L.ht;
+    R.ii = L.face;
+}
+
 /* ------------------------------------------------------------------ hop start */
-DSMC_HD void lane_begin_flight(DsmcLane &L, const DsmcEnv &E)
+/* Starts a hop from the surface.  The cell indices are walked from the cell the
+ * lane was last in (`near`: landing) or from a fixed guess (launch); the walks
+ * and the sector test are exact, so both give the same cell. */
+DSMC_HD void lane_begin_flight(DsmcLane &L, const DsmcEnv &E, bool near = false)
 {
     double Ri = sqrt(L.sx * L.sx + L.sy * L.sy + L.sz * L.sz);
     L.ox = E.GM * L.sx / (Ri * Ri * Ri);
@@ -411,9 +476,15 @@ DSMC_HD void lane_begin_flight(DsmcLane &L, const DsmcEnv &E)
     L.ii = 0;
     /* current cell guesses for the incremental walks */
     double q = L.sz / Ri;
-    L.td = table_walk(E.tq, E.ftd / 2, E.ftd - 1, q);
-    L.ht = table_walk(E.hr, 0, E.hd, Ri);
-    L.fd = fd_exact(L.sx, L.sy, E.ffd);
+    if (near) {
+        L.td = table_walk(E.tq, L.td, E.ftd - 1, q);
+        L.ht = table_walk(E.hr, L.ht, E.hd, Ri);
+        L.fd = fd_update(L.sx, L.sy, L.fd, E.ffd, E.cs);
+    } else {
+        L.td = table_walk(E.tq, E.ftd / 2, E.ftd - 1, q);
+        L.ht = table_walk(E.hr, 0, E.hd, Ri);
+        L.fd = fd_exact(L.sx, L.sy, E.ffd);
+    }
     L.phase = LANE_FLIGHT;
 }
 
@@ -546,13 +617,16 @@ DSMC_HD int locate_face(const DsmcEnv &E, double dx, double dy, double dz)
 {
     int best = 0;
     double bm = face_margin(E, 0, dx, dy, dz);
+#pragma unroll 1
     for (int f = 1; f < 20; f++) {
         double m = face_margin(E, f, dx, dy, dz);
         if (m > bm) { bm = m; best = f; }
     }
+#pragma unroll 1
     for (int l = 1; l <= E.levels; l++) {
         int base = 4 * best, pick = base;
         double pm = face_margin(E, E.level_off[l] + base, dx, dy, dz);
+#pragma unroll 1
         for (int j = 1; j < 4; j++) {
             double m = face_margin(E, E.level_off[l] + base + j, dx, dy, dz);
             if (m > pm) { pm = m; pick = base + j; }
@@ -560,6 +634,22 @@ DSMC_HD int locate_face(const DsmcEnv &E, double dx, double dy, double dz)
         best = pick;
     }
     return best;
+}
+
+/* locate_face with a guess: a direction at least 1e-12 |d| inside face `hint`
+ * (unit edge normals) is inside every ancestor of it by at least as much, and
+ * outside every other face of each level, so the hierarchical search would
+ * return `hint`; otherwise the search runs. */
+DSMC_HD int locate_face_near(const DsmcEnv &E, int hint, double dx, double dy, double dz, double dlen)
+{
+    if (hint >= 0 && hint < E.nfine && E.fnorm) {
+        const double *u = E.fnorm + (size_t)hint * 9;
+        const double eps = 1e-12 * dlen;
+        if (dx * u[0] + dy * u[1] + dz * u[2] >= eps && dx * u[3] + dy * u[4] + dz * u[5] >= eps &&
+            dx * u[6] + dy * u[7] + dz * u[8] >= eps)
+            return hint;
+    }
+    return locate_face(E, dx, dy, dz);
 }
 
 /* ------------------------------------------------------------------ particle start */
@@ -624,6 +714,7 @@ DSMC_HD void lane_start(DsmcLane &L, const DsmcEnv &E, int64_t pid, DsmcCounts &
     double Ex = -dm_log(lane_u01(L, E));
     double nph = ceil(Ex * E.lifetime / E.h);
     L.n_photo = nph > 9.0e18 ? (int64_t)9000000000000000000LL : (int64_t)nph;
+    L.face = face;
     lane_emit(L, E, E.face_T[face]);
     L.hops = 1;
     lane_begin_flight(L, E);
@@ -635,7 +726,7 @@ template <class Ops, int MODE, int ADV>
 DSMC_HD void lane_service(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
 {
     lane_flush<Ops>(L, E);
-    if (L.phase == LANE_ENDED) { lane_finish(L, E, cnt); return; }
+    if (L.phase == LANE_ENDED) { lane_finish<MODE>(L, E, cnt); return; }
     if (MODE == 0) {                         /* ballistic.c:239-256 / advanced.c:405-420 */
         double Ri = sqrt(L.sx * L.sx + L.sy * L.sy + L.sz * L.sz);
         pos_angles(L.sx, L.sy, L.sz, Ri, &L.thi, &L.fi);
@@ -649,24 +740,26 @@ DSMC_HD void lane_service(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
         return;
     }
     int64_t cap = E.max_steps > 0 ? E.max_steps : INT64_MAX;
-    if (L.steps >= cap) { L.fate = 5; lane_finish(L, E, cnt); return; }
-    int f = locate_face(E, L.sx, L.sy, L.sz);
+    if (L.steps >= cap) { L.fate = 5; lane_finish<MODE>(L, E, cnt); return; }
+    const double Rl = sqrt(L.sx * L.sx + L.sy * L.sy + L.sz * L.sz);
+    int f = locate_face_near(E, L.face, L.sx, L.sy, L.sz, Rl);
     double Tf = E.face_T[f];
     if (Tf < E.t_freeze) {
-        double Ri = sqrt(L.sx * L.sx + L.sy * L.sy + L.sz * L.sz), thi, fi;
-        pos_angles(L.sx, L.sy, L.sz, Ri, &thi, &fi);
+        double thi, fi;
+        pos_angles(L.sx, L.sy, L.sz, Rl, &thi, &fi);
         int td, fd;
         sph_index(thi, fi, E.ftd, E.ffd, &td, &fd);
         Ops::add_f64(&E.horizon[td * E.ffd + fd], L.W);
         L.fate = 2;
-        lane_finish(L, E, cnt);
+        lane_finish<MODE>(L, E, cnt);
         return;
     }
-    double sc = E.R / sqrt(L.sx * L.sx + L.sy * L.sy + L.sz * L.sz);
+    double sc = E.R / Rl;
     L.sx *= sc; L.sy *= sc; L.sz *= sc;
+    L.face = f;
     lane_emit(L, E, Tf);
     L.hops++;
-    lane_begin_flight(L, E);
+    lane_begin_flight(L, E, true);
 }
 
 /* ------------------------------------------------------------------ collision (advanced) */
@@ -694,11 +787,15 @@ DSMC_HD double sample_speed(const double *c, int deg, double min_v, double max_v
     return (v0 + v1) * 0.5;
 }
 
+template <class Ops, int ADV>
+DSMC_HD void lane_step_compat(DsmcLane &L, const DsmcEnv &E, const DsmcCellTabs &TB, double nx, double ny,
+                              double nz, double Rn, double ds);
+
 /* ------------------------------------------------------------------ one flight step */
 /* integrator.c:4-55 followed by ballistic.c:195-236 / advanced.c:347-402, or the
  * physical-mode step.  Returns with L.phase = FLIGHT, LANDED or DONE. */
-template <class Ops, int MODE, int ADV>
-DSMC_HD void lane_step(DsmcLane &L, const DsmcEnv &E, const DsmcCellTabs &TB)
+template <class Ops, int MODE, int ADV, class EnvT>
+DSMC_HD void lane_step(DsmcLane &L, const EnvT &E, const DsmcCellTabs &TB)
 {
     const double h = E.h;
     if (MODE == 0) { L.flight_time += h; L.time += h; }
@@ -725,7 +822,7 @@ DSMC_HD void lane_step(DsmcLane &L, const DsmcEnv &E, const DsmcCellTabs &TB)
     if (Rn <= E.R) { L.phase = LANE_LANDED; return; }
     if (Rn >= E.max_height) { lane_flush<Ops>(L, E); L.fate = 1; L.phase = LANE_ENDED; return; }
 
-    if (MODE != 0) {
+    if constexpr (MODE != 0) {
         if (nx * E.sun_x + ny * E.sun_y > 0.0) {
             L.n_day++;
             if (L.n_day >= L.n_photo) { lane_flush<Ops>(L, E); L.fate = 3; L.phase = LANE_ENDED; return; }
@@ -736,9 +833,16 @@ DSMC_HD void lane_step(DsmcLane &L, const DsmcEnv &E, const DsmcCellTabs &TB)
         if (L.ht < E.hd) lane_deposit<Ops>(L, E, (L.ht * E.ftd + L.td) * E.ffd + L.fd);
         int64_t cap = E.max_steps > 0 ? E.max_steps : INT64_MAX;
         if (L.steps >= cap) { lane_flush<Ops>(L, E); L.fate = 5; L.phase = LANE_ENDED; }
-        return;
+    } else {
+        lane_step_compat<Ops, ADV>(L, E, TB, nx, ny, nz, Rn, ds);
     }
+}
 
+/* The rest of a compat step (ballistic.c:195-236 / advanced.c:347-402). */
+template <class Ops, int ADV>
+DSMC_HD void lane_step_compat(DsmcLane &L, const DsmcEnv &E, const DsmcCellTabs &TB, double nx, double ny,
+                              double nz, double Rn, double ds)
+{
     L.td = td_update(nz, Rn, L.td, E.ftd, TB.tq);
     L.fd = fd_update(nx, ny, L.fd, E.ffd, TB.cs);
     if (L.fd < E.ffd / 2) L.W *= E.decay;

@@ -79,7 +79,7 @@ inline int ht_of_r(double Ri, double R, double unih, double alpha2, int exp_grid
 struct HostTables {
     std::vector<double> tq, hr, cs, vt_speeds;
     std::vector<int32_t> vt_count;
-    std::vector<double> cdf, enorm;
+    std::vector<double> cdf, enorm, fnorm;
     int level_off[12];
     int levels = 0, nfine = 0, nface_all = 0;
 };
@@ -213,6 +213,16 @@ inline int build_env(const dsmc_params *p, const dsmc_surface *s, int advanced, 
                 n[2] = a[0] * b[1] - a[1] * b[0];
             }
         }
+        /* unit normals of the finest faces (the locate_face_near guess test) */
+        T->fnorm.assign((size_t)T->nfine * 9, 0.0);
+        const size_t f0 = (size_t)T->level_off[T->levels];
+        for (size_t f = 0; f < (size_t)T->nfine; f++)
+            for (int e = 0; e < 3; e++) {
+                const double *n = &T->enorm[(f0 + f) * 9 + 3 * e];
+                double len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                if (!(len > 0)) return DSMC_ERR_ARG;
+                for (int j = 0; j < 3; j++) T->fnorm[f * 9 + 3 * e + j] = n[j] / len;
+            }
         for (int l = 0; l <= T->levels; l++) E->level_off[l] = T->level_off[l];
         E->levels = T->levels;
         E->nfine = T->nfine;

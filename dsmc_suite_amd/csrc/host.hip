@@ -110,11 +110,13 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
     } else {
         UP(rd->cdf, T.cdf);
         UP(rd->enorm, T.enorm);
+        UP(rd->fnorm, T.fnorm);
         if ((rc = rd->vert.upload(s->vert, sizeof(double) * 3 * s->nvert, st))) return rc;
         if ((rc = rd->faces.upload(s->faces, sizeof(int32_t) * 3 * T.nface_all, st))) return rc;
         if ((rc = rd->face_T.upload(s->face_T, sizeof(double) * T.nfine, st))) return rc;
         E.cdf = (const double *)rd->cdf.ptr;
         E.enorm = (const double *)rd->enorm.ptr;
+        E.fnorm = (const double *)rd->fnorm.ptr;
         E.vert = (const double *)rd->vert.ptr;
         E.faces = (const int32_t *)rd->faces.ptr;
         E.face_T = (const double *)rd->face_T.ptr;
@@ -201,9 +203,7 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
     int rc = device_props(&cus);
     if (rc) return rc;
     dsmc_launch_shape sh = shape ? *shape : default_shape(rd->E);
-    if (sh.block != 256 || sh.blocks_per_cu <= 0 || sh.blocks_per_cu > 64 || sh.relaunch_batch < 1 ||
-        sh.relaunch_batch > 64)
-        return DSMC_ERR_ARG;
+    if (!shape_ok(sh)) return DSMC_ERR_ARG;
     const int64_t n = rd->E.nparticles;
     DsmcQueues Q;
     Q.fly[0] = (int32_t *)rd->flyq0.ptr;
@@ -217,13 +217,14 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
     if (n == 0) return DSMC_OK;
     int64_t blocks = (n + 255) / 256;
     int start_grid = (int)(blocks < (int64_t)cus * 8 ? blocks : (int64_t)cus * 8);
-    HIP_OK(dsmc_launch_start(&rd->E, nullptr, nullptr, 1, rec, 0, n, Q, 0, start_grid, st));
+    HIP_OK(dsmc_launch_start(&rd->E, 0, rec, 0, n, Q, 0, start_grid, st));
     int flight_grid = cus * sh.blocks_per_cu;
     if ((int64_t)flight_grid > blocks) flight_grid = (int)blocks;
     int service_grid = start_grid;
     /* per-launch step budget: long hops are split over rounds so the run's tail is
      * set by its longest particle, not by the sum of each round's longest hop */
-    const int64_t sh_budget = rd->E.mode == 0 ? 4096 : 1024;
+    const int64_t sh_budget = sh.step_budget;
+    const bool inline_svc = sh.service_inline && rd->E.mode == DSMC_MODE_PHYSICAL && !rd->E.advanced;
     std::vector<hipEvent_t> ev;
     int p = 0, chunk = 4;
     for (;;) {
@@ -236,10 +237,16 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
                 ev.push_back(b);
                 HIP_OK(hipEventRecord(a, st));
             }
-            HIP_OK(dsmc_launch_flight(&rd->E, nullptr, rec, Q, p, flight_grid, sh.relaunch_batch,
-                                      sh_budget, st));
-            if (timed) HIP_OK(hipEventRecord(ev.back(), st));
-            HIP_OK(dsmc_launch_service(&rd->E, nullptr, rec, Q, p, service_grid, st));
+            if (inline_svc) {
+                HIP_OK(dsmc_launch_phys(&rd->E, nullptr, rec, Q, p, flight_grid, sh.block, sh.relaunch_batch,
+                                        sh_budget, st));
+                if (timed) HIP_OK(hipEventRecord(ev.back(), st));
+            } else {
+                HIP_OK(dsmc_launch_flight(&rd->E, rec, Q, p, flight_grid, sh.block, sh.relaunch_batch, sh_budget,
+                                          st));
+                if (timed) HIP_OK(hipEventRecord(ev.back(), st));
+                HIP_OK(dsmc_launch_service(&rd->E, rec, Q, p, service_grid, st));
+            }
             p ^= 1;
         }
         HIP_OK(hipMemcpyAsync(rd->host_ctl, Q.ctl, sizeof(unsigned int) * 16, hipMemcpyDeviceToHost, st));
@@ -267,7 +274,16 @@ dsmc_launch_shape default_shape(const DsmcEnv &E)
     s.block = 256;
     s.blocks_per_cu = 4;
     s.relaunch_batch = E.mode == 0 ? 8 : 16;
+    s.step_budget = E.mode == 0 ? 4096 : 1024;
+    s.service_inline = E.mode == DSMC_MODE_PHYSICAL ? 1 : 0;
     return s;
+}
+
+bool shape_ok(const dsmc_launch_shape &s)
+{
+    return s.block >= 64 && s.block <= 256 && s.block % 64 == 0 && s.blocks_per_cu > 0 &&
+           s.blocks_per_cu * s.block <= 2048 && s.relaunch_batch >= 1 && s.relaunch_batch <= 64 &&
+           s.step_budget >= 16 && (s.service_inline == 0 || s.service_inline == 1);
 }
 
 int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st)

@@ -19,17 +19,23 @@
  *
  * The flight kernel holds no relaunch code, which keeps its register footprint -
  * and so its occupancy - set by the Verlet step alone.  Every particle's
- * arithmetic is the per-lane state machine of dsmc_lane.h; runs of a batch share
- * the queues and carry their own environment (BATCH template).
+ * arithmetic is the per-lane state machine of dsmc_lane.h.
+ *
+ * Pools: many runs share the particle slots and the queues (rec.run names the
+ * run).  Pool flight lanes load their run's step constants (DsmcStepEnv) into
+ * registers when they pick up a particle, so a wavefront can hold particles of
+ * different runs; the per-body radial thresholds of every run live in LDS.
  */
 #include <hip/hip_runtime.h>
 #include "dsmc_lane.h"
 #include "dsmc_kernels.h"
+#include <string.h>
 
 static_assert(sizeof(DsmcRec) == 256, "particle record must stay 256 bytes");
 
 /* Deposits straight into the device grid (global FP64 atomics). */
-__device__ static inline void grid_deposit(const DsmcEnv &E, int cell, double w, double vx, double vy,
+template <class EnvT>
+__device__ static inline void grid_deposit(const EnvT &E, int cell, double w, double vx, double vy,
                                            double vz, int n)
 {
     if (!E.density_from_count) atomicAdd(&E.density[cell], w);
@@ -42,7 +48,8 @@ __device__ static inline void grid_deposit(const DsmcEnv &E, int cell, double w,
 }
 
 struct DsmcOpsDev {
-    __device__ static inline void deposit(const DsmcEnv &E, int cell, double w, double vx, double vy,
+    template <class EnvT>
+    __device__ static inline void deposit(const EnvT &E, int cell, double w, double vx, double vy,
                                           double vz, int n)
     {
         grid_deposit(E, cell, w, vx, vy, vz, n);
@@ -66,7 +73,8 @@ __shared__ double dep_w[DEP_SLOTS], dep_vx[DEP_SLOTS], dep_vy[DEP_SLOTS], dep_vz
 __device__ static inline int dep_slot(int cell) { return (int)(((uint32_t)cell * 2654435761u) >> 23); }
 
 struct DsmcOpsLds {
-    __device__ static inline void deposit(const DsmcEnv &E, int cell, double w, double vx, double vy,
+    template <class EnvT>
+    __device__ static inline void deposit(const EnvT &E, int cell, double w, double vx, double vy,
                                           double vz, int n)
     {
         int s = dep_slot(cell);
@@ -118,53 +126,42 @@ __device__ static inline void wave_append(bool pred, int32_t slot, int32_t *q, u
     if (pred) q[base + __popcll(m & lanemask_lt())] = slot;
 }
 
-template <bool BATCH>
-__device__ static inline const DsmcEnv &env_of(const DsmcEnv &E0, const DsmcEnv *envs, int run)
-{
-    return BATCH ? envs[run] : E0;
-}
-
 /* ------------------------------------------------------------------ start */
-/* Initialises slots [0, nslots): slot -> (run, pid) through run_base. */
-template <int MODE, int ADV, bool BATCH>
+/* Starts particles [0, n) of one run in slots slot0..slot0+n-1. */
+template <int MODE, int ADV>
 __global__ void __launch_bounds__(256)
-start_kernel(DsmcEnv E0, const DsmcEnv *envs, const int64_t *run_base, int nruns, DsmcRec *rec,
-             int64_t slot0, int64_t nslots, int32_t *flyq, unsigned int *fly_count)
+start_kernel(DsmcEnv E, int32_t run_id, DsmcRec *rec, int64_t slot0, int64_t n, int32_t *flyq,
+             unsigned int *fly_count)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < nslots; b += stride) {
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < n; b += stride) {
         int64_t i = b + __lane_id();
         bool fly = false;
         int64_t slot = slot0 + i;
-        if (i < nslots) {
-            int run = 0;
-            if (BATCH) while (run + 1 < nruns && slot >= run_base[run + 1]) run++;
-            const DsmcEnv &E = env_of<BATCH>(E0, envs, run);
+        if (i < n) {
             DsmcLane L;
-            DsmcCounts cnt = {0, 0, 0, 0, 0, 0, 0};
-            lane_start<DsmcOpsDev, MODE, ADV>(L, E, slot - (BATCH ? run_base[run] : slot0), cnt);
+            DsmcCounts cnt = {0, 0, 0, 0, 0, 0, 0, 0};
+            lane_start<DsmcOpsDev, MODE, ADV>(L, E, i, cnt);
             counts_flush<DsmcOpsDev>(cnt, E);
             fly = L.phase == LANE_FLIGHT;
-            if (fly) rec_store_all(L, rec[slot], run);
+            if (fly) rec_store_all<MODE>(L, rec[slot], run_id);
         }
         wave_append(fly, (int32_t)slot, flyq, fly_count);
     }
 }
 
 /* ------------------------------------------------------------------ flight */
-template <int MODE, int ADV, bool BATCH>
+template <int MODE, int ADV>
 __global__ void __launch_bounds__(256)
-flight_kernel(DsmcEnv E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch,
-              int64_t step_budget)
+flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch, int64_t step_budget)
 {
-    /* per-step tables -> LDS (single-run launches; a batch reads each run's own) */
+    /* per-step tables and the deposit slot table -> LDS */
     extern __shared__ double lds_tabs[];
     DsmcCellTabs TB;
-    if (!BATCH) {
-        const int ntq = E0.ftd + 1, nhr = E0.hd + 2, ncs = 2 * (E0.ffd + 1);
+    {
+        const int ntq = E.ftd + 1, nhr = E.hd + 2, ncs = 2 * (E.ffd + 1);
         for (int i = threadIdx.x; i < ntq + nhr + ncs; i += blockDim.x)
-            lds_tabs[i] = i < ntq ? E0.tq[i] : (i < ntq + nhr ? E0.hr[i - ntq] : E0.cs[i - ntq - nhr]);
-        __syncthreads();
+            lds_tabs[i] = i < ntq ? E.tq[i] : (i < ntq + nhr ? E.hr[i - ntq] : E.cs[i - ntq - nhr]);
         TB.tq = lds_tabs;
         TB.hr = lds_tabs + ntq;
         TB.cs = lds_tabs + ntq + nhr;
@@ -177,7 +174,7 @@ flight_kernel(DsmcEnv E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int p
     const uint32_t lane = __lane_id();
     DsmcLane L;
     L.phase = LANE_IDLE;
-    int32_t slot = -1, run = 0;
+    int32_t slot = -1;
     int64_t budget_end = 0;
     bool exhausted = false;
     for (;;) {
@@ -194,7 +191,6 @@ flight_kernel(DsmcEnv E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int p
                 unsigned int qi = base + __popcll(idle & lanemask_lt());
                 if (qi < nq) {
                     slot = flyq[qi];
-                    if (BATCH) run = rec[slot].run;
                     rec_load_flight<MODE, ADV>(L, rec[slot]);
                     budget_end = step_budget > 0 ? L.steps + step_budget : INT64_MAX;
                 }
@@ -205,13 +201,10 @@ flight_kernel(DsmcEnv E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int p
             if (exhausted) break;
             continue;
         }
-        /* step until some lanes have ended their hop */
-        const DsmcEnv &E = env_of<BATCH>(E0, envs, run);
-        if (BATCH) { TB.tq = E.tq; TB.hr = E.hr; TB.cs = E.cs; }
+        /* step until enough lanes have ended their hop */
         for (;;) {
             if (L.phase == LANE_FLIGHT) {
-                if (BATCH) lane_step<DsmcOpsDev, MODE, ADV>(L, E, TB);
-                else lane_step<DsmcOpsLds, MODE, ADV>(L, E, TB);
+                lane_step<DsmcOpsLds, MODE, ADV>(L, E, TB);
                 if (L.phase == LANE_FLIGHT && L.steps >= budget_end) L.phase = LANE_IDLE + 8;
             }
             uint64_t ended = __ballot(L.phase != LANE_FLIGHT && L.phase != LANE_IDLE);
@@ -221,8 +214,7 @@ flight_kernel(DsmcEnv E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int p
         bool hop_end = L.phase == LANE_LANDED || L.phase == LANE_ENDED;
         bool paused = L.phase == LANE_IDLE + 8;
         if (hop_end || paused) {
-            if (BATCH) lane_flush<DsmcOpsDev>(L, E);
-            else lane_flush<DsmcOpsLds>(L, E);
+            lane_flush<DsmcOpsLds>(L, E);
             if (paused) L.phase = LANE_FLIGHT;
             rec_store_flight<MODE, ADV>(L, rec[slot]);
         }
@@ -230,14 +222,205 @@ flight_kernel(DsmcEnv E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int p
         wave_append(paused, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
         if (hop_end || paused) L.phase = LANE_IDLE;
     }
-    if (!BATCH) {
+    __syncthreads();
+    dep_merge(E);
+}
+
+/* Flight kernel of a pool: as above, with each lane's run constants in registers
+ * and every run's radial thresholds in LDS (deposits go straight to the run's
+ * grid; the LDS slot table is for single runs). */
+template <int MODE, int ADV>
+__global__ void __launch_bounds__(256)
+pool_flight_kernel(DsmcPoolDev P, DsmcQueues Q, int p, int refill_batch, int64_t step_budget)
+{
+    static_assert(MODE == 1 && ADV == 0, "pools run physical ballistic particles");
+    extern __shared__ double lds_tabs[];
+    const int ntq = P.ftd + 1, ncs = 2 * (P.ffd + 1);
+    for (int i = threadIdx.x; i < ntq + ncs + P.nhr; i += blockDim.x)
+        lds_tabs[i] = i < ntq ? P.tq[i] : (i < ntq + ncs ? P.cs[i - ntq] : P.hr[i - ntq - ncs]);
+    __syncthreads();
+    const double *lds_hr = lds_tabs + ntq + ncs;
+    DsmcCellTabs TB;
+    TB.tq = lds_tabs;
+    TB.cs = lds_tabs + ntq;
+    TB.hr = lds_hr;
+
+    const unsigned int nq = Q.ctl[2 * p];
+    unsigned int *head = &Q.ctl[2 * p + 1];
+    const int32_t *flyq = Q.fly[p];
+    DsmcRec *rec = P.rec;
+    const uint32_t lane = __lane_id();
+    DsmcLane L;
+    L.phase = LANE_IDLE;
+    DsmcStepEnv S = {};
+    int32_t slot = -1;
+    int64_t budget_end = 0;
+    bool exhausted = false;
+    for (;;) {
+        uint64_t idle = __ballot(L.phase == LANE_IDLE);
+        uint64_t flying = ~idle & __ballot(true);
+        if (!exhausted && idle && (__popcll(idle) >= refill_batch || flying == 0)) {
+            int leader = __ffsll((long long)idle) - 1;
+            unsigned int base = 0;
+            if ((int)lane == leader) base = atomicAdd(head, (unsigned int)__popcll(idle));
+            base = __shfl(base, leader);
+            if (base + (unsigned int)__popcll(idle) >= nq) exhausted = true;
+            if (idle & (1ull << lane)) {
+                unsigned int qi = base + __popcll(idle & lanemask_lt());
+                if (qi < nq) {
+                    slot = flyq[qi];
+                    S = P.steps[rec[slot].run];
+                    TB.hr = lds_hr + S.hr_off;
+                    rec_load_flight<MODE, ADV>(L, rec[slot]);
+                    budget_end = step_budget > 0 ? L.steps + step_budget : INT64_MAX;
+                }
+            }
+            flying = __ballot(L.phase == LANE_FLIGHT);
+        }
+        if (flying == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        S.ftd = P.ftd; S.ffd = P.ffd; S.hd = P.hd;    /* pool-wide: kept scalar */
+        for (;;) {
+            if (L.phase == LANE_FLIGHT) {
+                lane_step<DsmcOpsDev, MODE, ADV>(L, S, TB);
+                if (L.phase == LANE_FLIGHT && L.steps >= budget_end) L.phase = LANE_IDLE + 8;
+            }
+            uint64_t ended = __ballot(L.phase != LANE_FLIGHT && L.phase != LANE_IDLE);
+            if (ended && (__popcll(ended | idle) >= refill_batch || __ballot(L.phase == LANE_FLIGHT) == 0)) break;
+        }
+        bool hop_end = L.phase == LANE_LANDED || L.phase == LANE_ENDED;
+        bool paused = L.phase == LANE_IDLE + 8;
+        if (hop_end || paused) {
+            lane_flush<DsmcOpsDev>(L, S);
+            if (paused) L.phase = LANE_FLIGHT;
+            rec_store_flight<MODE, ADV>(L, rec[slot]);
+        }
+        wave_append(hop_end, slot, Q.svc, &Q.ctl[4]);
+        wave_append(paused, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
+        if (hop_end || paused) L.phase = LANE_IDLE;
+    }
+}
+
+/* The physical landing service, out of line: the flight kernel's registers stay
+ * those of the Verlet step; the lane state travels through the call frame. */
+template <class Ops>
+__device__ __attribute__((noinline)) void phys_service(DsmcLane &T, const DsmcEnv &E, DsmcCounts &cnt)
+{
+    lane_service<Ops, 1, 0>(T, E, cnt);
+}
+
+/* Physical particles: flight with the landing service inline.  A lane whose hop
+ * ends lands (freeze-out or re-emission) or finishes its particle in the same
+ * launch, so a particle makes as many hops per launch as its step budget
+ * allows; only budget-paused particles wait for the next launch.  Lanes whose
+ * hops ended wait (masked) until `svc_batch` lanes of the wavefront need the
+ * service, so the service code runs for many lanes at once.  POOL: lanes hold
+ * particles of different runs (step constants in registers, every run's radial
+ * thresholds in LDS); otherwise one run's environment is the kernel argument and
+ * deposits go through the LDS slot table. */
+template <bool POOL>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+phys_kernel(DsmcEnv E0, DsmcPoolDev P, DsmcRec *rec, DsmcQueues Q, int p, int svc_batch, int64_t step_budget)
+{
+    extern __shared__ double lds_tabs[];
+    const int ftd = POOL ? P.ftd : E0.ftd, ffd = POOL ? P.ffd : E0.ffd, hd = POOL ? P.hd : E0.hd;
+    const int ntq = ftd + 1, ncs = 2 * (ffd + 1), nhr = POOL ? P.nhr : hd + 2;
+    const double *gtq = POOL ? P.tq : E0.tq, *gcs = POOL ? P.cs : E0.cs, *ghr = POOL ? P.hr : E0.hr;
+    for (int i = threadIdx.x; i < ntq + ncs + nhr; i += blockDim.x)
+        lds_tabs[i] = i < ntq ? gtq[i] : (i < ntq + ncs ? gcs[i - ntq] : ghr[i - ntq - ncs]);
+    if (!POOL) dep_init();
+    __syncthreads();
+    const double *lds_hr = lds_tabs + ntq + ncs;
+    DsmcCellTabs TB;
+    TB.tq = lds_tabs;
+    TB.cs = lds_tabs + ntq;
+    TB.hr = lds_hr;
+    if (POOL) rec = P.rec;
+
+    const unsigned int nq = Q.ctl[2 * p];
+    unsigned int *head = &Q.ctl[2 * p + 1];
+    const int32_t *flyq = Q.fly[p];
+    const uint32_t lane = __lane_id();
+    DsmcLane L;
+    L.phase = LANE_IDLE;
+    DsmcStepEnv S = {};
+    int32_t slot = -1, run = 0;
+    int64_t budget_end = 0;
+    bool exhausted = false;
+    for (;;) {
+        uint64_t idle = __ballot(L.phase == LANE_IDLE);
+        uint64_t flying = ~idle & __ballot(true);
+        if (!exhausted && idle && (__popcll(idle) >= svc_batch || flying == 0)) {
+            int leader = __ffsll((long long)idle) - 1;
+            unsigned int base = 0;
+            if ((int)lane == leader) base = atomicAdd(head, (unsigned int)__popcll(idle));
+            base = __shfl(base, leader);
+            if (base + (unsigned int)__popcll(idle) >= nq) exhausted = true;
+            if (idle & (1ull << lane)) {
+                unsigned int qi = base + __popcll(idle & lanemask_lt());
+                if (qi < nq) {
+                    slot = flyq[qi];
+                    if (POOL) {
+                        run = rec[slot].run;
+                        S = P.steps[run];
+                        TB.hr = lds_hr + S.hr_off;
+                    }
+                    rec_load_phys(L, rec[slot]);
+                    budget_end = step_budget > 0 ? L.steps + step_budget : INT64_MAX;
+                }
+            }
+            flying = __ballot(L.phase == LANE_FLIGHT);
+        }
+        if (flying == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (POOL) { S.ftd = P.ftd; S.ffd = P.ffd; S.hd = P.hd; }
+        for (;;) {
+            if (L.phase == LANE_FLIGHT) {
+                if (POOL) lane_step<DsmcOpsDev, 1, 0>(L, S, TB);
+                else lane_step<DsmcOpsLds, 1, 0>(L, E0, TB);
+                if (L.phase == LANE_FLIGHT && L.steps >= budget_end) L.phase = LANE_IDLE + 8;
+            }
+            uint64_t ended = __ballot(L.phase != LANE_FLIGHT && L.phase != LANE_IDLE);
+            uint64_t waiting = ended | __ballot(L.phase == LANE_IDLE);
+            if ((ended && __popcll(waiting) >= svc_batch) || waiting == __ballot(true)) break;
+        }
+        /* landings and ended particles: service inline */
+        if (L.phase == LANE_LANDED || L.phase == LANE_ENDED) {
+            const DsmcEnv &E = POOL ? P.envs[run] : E0;
+            DsmcCounts cnt = {0, 0, 0, 0, 0, 0, 0, 0};
+            DsmcLane T = L;
+            if (POOL) phys_service<DsmcOpsDev>(T, E, cnt);
+            else phys_service<DsmcOpsLds>(T, E, cnt);
+            L = T;
+            if (L.phase == LANE_DONE) {
+                counts_flush<DsmcOpsDev>(cnt, E);
+                L.phase = LANE_IDLE;
+            }
+        }
+        /* budget-paused particles go to the next launch */
+        bool paused = L.phase == LANE_IDLE + 8;
+        if (paused) {
+            if (POOL) lane_flush<DsmcOpsDev>(L, S);
+            else lane_flush<DsmcOpsLds>(L, E0);
+            L.phase = LANE_FLIGHT;
+            rec_store_phys(L, rec[slot]);
+        }
+        wave_append(paused, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
+        if (paused) L.phase = LANE_IDLE;
+    }
+    if (!POOL) {
         __syncthreads();
         dep_merge(E0);
     }
 }
 
 /* ------------------------------------------------------------------ service */
-template <int MODE, int ADV, bool BATCH>
+/* One lane per queued slot.  POOL: the record names its run's environment. */
+template <int MODE, int ADV, bool POOL>
 __global__ void __launch_bounds__(256)
 service_kernel(DsmcEnv E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int p)
 {
@@ -249,18 +432,24 @@ service_kernel(DsmcEnv E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int 
         int32_t slot = 0;
         if (i < n) {
             slot = Q.svc[i];
-            int run = BATCH ? rec[slot].run : 0;
-            const DsmcEnv &E = env_of<BATCH>(E0, envs, run);
+            int run = POOL ? rec[slot].run : 0;
+            const DsmcEnv &E = POOL ? envs[run] : E0;
             DsmcLane L;
-            rec_load_all(L, rec[slot]);
-            DsmcCounts cnt = {0, 0, 0, 0, 0, 0, 0};
+            rec_load_all<MODE>(L, rec[slot]);
+            DsmcCounts cnt = {0, 0, 0, 0, 0, 0, 0, 0};
             lane_service<DsmcOpsDev, MODE, ADV>(L, E, cnt);
             counts_flush<DsmcOpsDev>(cnt, E);
             fly = L.phase == LANE_FLIGHT;
-            if (fly) rec_store_all(L, rec[slot], run);
+            if (fly) rec_store_all<MODE>(L, rec[slot], run);
         }
         wave_append(fly, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
     }
+}
+
+__global__ void fill0_kernel(unsigned long long *p, size_t n)
+{
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0ull;
 }
 
 /* ------------------------------------------------------------------ reset */
@@ -326,44 +515,98 @@ __global__ void dsmc_math_kernel(int fn, const double *x, const double *y, doubl
 }
 
 /* ------------------------------------------------------------------ launchers */
-#define DISPATCH(KERNEL, ...)                                                                  \
-    do {                                                                                       \
-        if (mode == 0 && !adv && !batch) hipLaunchKernelGGL((KERNEL<0, 0, false>), __VA_ARGS__); \
-        else if (mode == 0 && !adv) hipLaunchKernelGGL((KERNEL<0, 0, true>), __VA_ARGS__);     \
-        else if (mode == 0 && !batch) hipLaunchKernelGGL((KERNEL<0, 1, false>), __VA_ARGS__);  \
-        else if (mode == 0) hipLaunchKernelGGL((KERNEL<0, 1, true>), __VA_ARGS__);             \
-        else if (!batch) hipLaunchKernelGGL((KERNEL<1, 0, false>), __VA_ARGS__);               \
-        else hipLaunchKernelGGL((KERNEL<1, 0, true>), __VA_ARGS__);                            \
+#define DISPATCH(KERNEL, ...)                                                                 \
+    do {                                                                                      \
+        if (mode == 0 && !adv) hipLaunchKernelGGL((KERNEL<0, 0>), __VA_ARGS__);               \
+        else if (mode == 0) hipLaunchKernelGGL((KERNEL<0, 1>), __VA_ARGS__);                  \
+        else if (!adv) hipLaunchKernelGGL((KERNEL<1, 0>), __VA_ARGS__);                       \
+        else return hipErrorInvalidValue;                                                     \
     } while (0)
 
-extern "C" hipError_t dsmc_launch_start(const DsmcEnv *E0, const DsmcEnv *envs, const int64_t *run_base,
-                                        int nruns, DsmcRec *rec, int64_t slot0, int64_t nslots,
+static bool block_ok(int block) { return block >= 64 && block <= 256 && block % 64 == 0; }
+
+extern "C" hipError_t dsmc_launch_start(const DsmcEnv *E, int32_t run_id, DsmcRec *rec, int64_t slot0, int64_t n,
                                         DsmcQueues Q, int p, int grid, hipStream_t st)
 {
-    const int mode = E0->mode, adv = E0->advanced;
-    const bool batch = envs != nullptr;
-    DISPATCH(start_kernel, dim3(grid), dim3(256), 0, st, *E0, envs, run_base, nruns, rec, slot0, nslots,
-             Q.fly[p], &Q.ctl[2 * p]);
+    const int mode = E->mode, adv = E->advanced;
+    DISPATCH(start_kernel, dim3(grid), dim3(256), 0, st, *E, run_id, rec, slot0, n, Q.fly[p], &Q.ctl[2 * p]);
     return hipGetLastError();
 }
 
-extern "C" hipError_t dsmc_launch_flight(const DsmcEnv *E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q,
-                                         int p, int grid, int refill_batch, int64_t step_budget, hipStream_t st)
+extern "C" hipError_t dsmc_launch_flight(const DsmcEnv *E, DsmcRec *rec, DsmcQueues Q, int p, int grid, int block,
+                                         int refill_batch, int64_t step_budget, hipStream_t st)
 {
-    const int mode = E0->mode, adv = E0->advanced;
-    const bool batch = envs != nullptr;
-    const size_t lds = batch ? 0 : sizeof(double) * (size_t)((E0->ftd + 1) + (E0->hd + 2) + 2 * (E0->ffd + 1));
-    DISPATCH(flight_kernel, dim3(grid), dim3(256), lds, st, *E0, envs, rec, Q, p, refill_batch, step_budget);
+    if (!block_ok(block)) return hipErrorInvalidValue;
+    const int mode = E->mode, adv = E->advanced;
+    const size_t lds = sizeof(double) * (size_t)((E->ftd + 1) + (E->hd + 2) + 2 * (E->ffd + 1));
+    DISPATCH(flight_kernel, dim3(grid), dim3(block), lds, st, *E, rec, Q, p, refill_batch, step_budget);
     return hipGetLastError();
 }
 
-extern "C" hipError_t dsmc_launch_service(const DsmcEnv *E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q,
-                                          int p, int grid, hipStream_t st)
+extern "C" hipError_t dsmc_launch_service(const DsmcEnv *E, DsmcRec *rec, DsmcQueues Q, int p, int grid,
+                                          hipStream_t st)
 {
-    const int mode = E0->mode, adv = E0->advanced;
-    const bool batch = envs != nullptr;
-    DISPATCH(service_kernel, dim3(grid), dim3(256), 0, st, *E0, envs, rec, Q, p);
+    const int mode = E->mode, adv = E->advanced;
+    if (mode == 0 && !adv) hipLaunchKernelGGL((service_kernel<0, 0, false>), dim3(grid), dim3(256), 0, st, *E,
+                                               nullptr, rec, Q, p);
+    else if (mode == 0) hipLaunchKernelGGL((service_kernel<0, 1, false>), dim3(grid), dim3(256), 0, st, *E,
+                                           nullptr, rec, Q, p);
+    else if (!adv) hipLaunchKernelGGL((service_kernel<1, 0, false>), dim3(grid), dim3(256), 0, st, *E,
+                                      nullptr, rec, Q, p);
+    else return hipErrorInvalidValue;
     hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, Q.ctl, p, Q.ctl + 8);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t dsmc_launch_pool_flight(const DsmcPoolDev *P, DsmcQueues Q, int p, int grid, int block,
+                                              int refill_batch, int64_t step_budget, hipStream_t st)
+{
+    if (!block_ok(block) || P->mode != 1 || P->adv) return hipErrorInvalidValue;
+    const size_t lds = sizeof(double) * (size_t)((P->ftd + 1) + 2 * (P->ffd + 1) + P->nhr);
+    hipLaunchKernelGGL((pool_flight_kernel<1, 0>), dim3(grid), dim3(block), lds, st, *P, Q, p, refill_batch,
+                       step_budget);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t dsmc_launch_phys(const DsmcEnv *E, const DsmcPoolDev *P, DsmcRec *rec, DsmcQueues Q, int p,
+                                       int grid, int block, int svc_batch, int64_t step_budget, hipStream_t st)
+{
+    if (!block_ok(block)) return hipErrorInvalidValue;
+    if (P) {
+        if (P->mode != 1 || P->adv) return hipErrorInvalidValue;
+        const size_t lds = sizeof(double) * (size_t)((P->ftd + 1) + 2 * (P->ffd + 1) + P->nhr);
+        DsmcEnv none;
+        memset(&none, 0, sizeof none);
+        hipLaunchKernelGGL((phys_kernel<true>), dim3(grid), dim3(block), lds, st, none, *P, rec, Q, p, svc_batch,
+                           step_budget);
+    } else {
+        if (E->mode != 1 || E->advanced) return hipErrorInvalidValue;
+        const size_t lds = sizeof(double) * (size_t)((E->ftd + 1) + 2 * (E->ffd + 1) + (E->hd + 2));
+        DsmcPoolDev none;
+        memset(&none, 0, sizeof none);
+        hipLaunchKernelGGL((phys_kernel<false>), dim3(grid), dim3(block), lds, st, *E, none, rec, Q, p, svc_batch,
+                           step_budget);
+    }
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, Q.ctl, p, Q.ctl + 8);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t dsmc_launch_pool_service(const DsmcPoolDev *P, DsmcQueues Q, int p, int grid, hipStream_t st)
+{
+    if (P->mode != 1 || P->adv) return hipErrorInvalidValue;
+    DsmcEnv none;
+    memset(&none, 0, sizeof none);
+    hipLaunchKernelGGL((service_kernel<1, 0, true>), dim3(grid), dim3(256), 0, st, none, P->envs, P->rec, Q, p);
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, Q.ctl, p, Q.ctl + 8);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t dsmc_launch_fill0(void *p, size_t n8, hipStream_t st)
+{
+    if (n8 == 0) return hipSuccess;
+    size_t blocks = (n8 + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(fill0_kernel, dim3((unsigned)blocks), dim3(256), 0, st, (unsigned long long *)p, n8);
     return hipGetLastError();
 }
 

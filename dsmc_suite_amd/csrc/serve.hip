@@ -1,0 +1,245 @@
+/*
+ * serve.hip - the continuous-batching request loop (dsmc_server_*).
+ *
+ * Requests are prepared on the host when submitted (environment, exact index
+ * tables), queued in arrival order and admitted into one device pool as soon as
+ * a contiguous range of particle slots, a grid slot and table slots are free:
+ * the pool's particle slots are the server's HBM budget divided by the slot
+ * footprint (256 B record + 12 B of queue entries, + 64 B with per-particle
+ * outputs), after the grid slots (max_inflight x grid bytes) are set aside.
+ * Admitted runs join the particles already in flight at the next replay of the
+ * round graph; a finished run's results are copied out on a second stream while
+ * the next replay runs, and its slots are free for the next request.
+ *
+ * Every particle's random stream is keyed by its run's seed and its index in the
+ * run, so a request's per-particle results do not depend on what else is in the
+ * pool; its grid sums differ only by the order of floating-point atomic
+ * additions (integer counts are exact).
+ */
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+#include <chrono>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "pool.h"
+
+using namespace dsmc;
+
+namespace {
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+enum { REQ_QUEUED = 0, REQ_ADMITTED = 1, REQ_DONE = 2 };
+
+struct ServeReq {
+    PoolPrep prep;
+    dsmc_result *r = nullptr;
+    double t_submit = 0.0;
+    int state = REQ_QUEUED, cancelled = 0, status = 0, g = -1;
+};
+
+}  // namespace
+
+struct dsmc_server {
+    size_t budget = 0;
+    int max_inflight = 0, per_particle = 0;
+    std::unique_ptr<Pool> pool;
+    PoolClass cls;
+    std::deque<uint64_t> queue;
+    std::map<uint64_t, ServeReq> reqs;
+    std::vector<uint64_t> ticket_of;   /* run slot -> ticket */
+    uint64_t next_ticket = 1;
+    int64_t completed = 0, steps_done = 0;
+};
+
+extern "C" dsmc_server *dsmc_server_create(size_t hbm_budget_bytes, int max_inflight, int per_particle)
+{
+    if (hbm_budget_bytes == 0 || max_inflight <= 0 || max_inflight > 4096) return nullptr;
+    if (ensure_device()) return nullptr;
+    dsmc_server *sv = new dsmc_server();
+    sv->budget = hbm_budget_bytes;
+    sv->max_inflight = max_inflight;
+    sv->per_particle = per_particle ? 1 : 0;
+    return sv;
+}
+
+static int make_pool(dsmc_server *sv, const PoolClass &c)
+{
+    size_t fixed = (size_t)sv->max_inflight * pool_run_bytes(c) + ((size_t)64 << 20);
+    if (sv->budget <= fixed) return DSMC_ERR_BUDGET;
+    int64_t slots = (int64_t)((sv->budget - fixed) / pool_slot_bytes(sv->per_particle));
+    if (slots > INT32_MAX) slots = INT32_MAX;
+    if (slots <= 0) return DSMC_ERR_BUDGET;
+    std::unique_ptr<Pool> P(new Pool());
+    int rc = pool_create(P.get(), c, sv->max_inflight, slots, sv->per_particle, nullptr);
+    if (rc) return rc;
+    sv->pool = std::move(P);
+    sv->cls = c;
+    sv->ticket_of.assign(sv->max_inflight, 0);
+    return DSMC_OK;
+}
+
+extern "C" int dsmc_server_set_shape(dsmc_server *sv, const dsmc_launch_shape *shape)
+{
+    if (!sv || !shape || !sv->pool) return DSMC_ERR_ARG;
+    if (sv->pool->pending || sv->pool->inflight) return DSMC_ERR_BUSY;
+    return pool_set_shape(sv->pool.get(), *shape);
+}
+
+extern "C" int dsmc_server_submit(dsmc_server *sv, const dsmc_params *p, const dsmc_surface *s, dsmc_result *r,
+                                  uint64_t *ticket)
+{
+    if (!sv || !p || !s || !r || !ticket) return DSMC_ERR_ARG;
+    PoolClass c;
+    int rc = pool_class_of(p, s, &c);
+    if (rc) return rc;
+    if (!sv->pool) {
+        if ((rc = make_pool(sv, c))) return rc;
+    } else if (c.ftd != sv->cls.ftd || c.ffd != sv->cls.ffd || c.hd != sv->cls.hd || c.levels != sv->cls.levels) {
+        return DSMC_ERR_ARG;
+    }
+    if (p->nparticles > sv->pool->max_slots) return DSMC_ERR_BUDGET;
+    if (!sv->per_particle && (r->end_state || r->steps || r->fate || r->hops)) return DSMC_ERR_ARG;
+    uint64_t t = sv->next_ticket++;
+    ServeReq &q = sv->reqs[t];
+    if ((rc = pool_prepare(p, s, &q.prep))) {
+        sv->reqs.erase(t);
+        return rc;
+    }
+    q.r = r;
+    q.t_submit = now_ms();
+    sv->queue.push_back(t);
+    *ticket = t;
+    return DSMC_OK;
+}
+
+static void finish(dsmc_server *sv, uint64_t t, ServeReq &q, int status)
+{
+    q.state = REQ_DONE;
+    q.status = status;
+    if (q.r) q.r->elapsed_ms = now_ms() - q.t_submit;
+    (void)sv;
+}
+
+/* Admits queued requests in arrival order while they fit, keeps two replays of
+ * the round graph in flight, and completes the runs the older one finished.
+ * Returns the number of requests completed, or a negative status. */
+extern "C" int dsmc_server_step(dsmc_server *sv)
+{
+    if (!sv) return DSMC_ERR_ARG;
+    if (!sv->pool) return 0;
+    Pool *P = sv->pool.get();
+    while (!sv->queue.empty()) {
+        uint64_t t = sv->queue.front();
+        ServeReq &q = sv->reqs[t];
+        int g = -1;
+        int rc = pool_admit(P, q.prep, t, &g);
+        if (rc == DSMC_ERR_BUSY) break;
+        sv->queue.pop_front();
+        if (rc) { finish(sv, t, q, rc); continue; }
+        q.state = REQ_ADMITTED;
+        q.g = g;
+        sv->ticket_of[g] = t;
+        q.prep.T = HostTables();            /* tables are on the device now */
+    }
+    while (P->inflight > 0 && P->pending < 2) {
+        int rc = pool_advance(P, 1);
+        if (rc) return rc;
+    }
+    if (P->pending == 0) return 0;
+    std::vector<int> fin;
+    int nf = pool_poll(P, &fin);
+    if (nf < 0) return nf;
+    int done = 0;
+    for (int g : fin) {
+        uint64_t t = sv->ticket_of[g];
+        ServeReq &q = sv->reqs[t];
+        int rc = q.cancelled ? DSMC_ERR_CANCELLED : pool_collect(P, g, q.r);
+        if (!q.cancelled && !rc) {
+            sv->completed++;
+            sv->steps_done += q.r->total_steps;
+        }
+        pool_release(P, g);
+        sv->ticket_of[g] = 0;
+        finish(sv, t, q, rc);
+        done++;
+    }
+    return done;
+}
+
+extern "C" int dsmc_server_poll(dsmc_server *sv, uint64_t ticket)
+{
+    if (!sv) return DSMC_ERR_ARG;
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end()) return DSMC_ERR_ARG;
+    if (it->second.state != REQ_DONE) return 0;
+    return it->second.status ? it->second.status : 1;
+}
+
+extern "C" int dsmc_server_wait(dsmc_server *sv, uint64_t ticket)
+{
+    for (;;) {
+        int st = dsmc_server_poll(sv, ticket);
+        if (st != 0) return st < 0 ? st : DSMC_OK;
+        int rc = dsmc_server_step(sv);
+        if (rc < 0) return rc;
+    }
+}
+
+/* A queued request is dropped at once; an admitted one finishes on the device
+ * (its particles hold their slots until then) and its results are discarded. */
+extern "C" int dsmc_server_cancel(dsmc_server *sv, uint64_t ticket)
+{
+    if (!sv) return DSMC_ERR_ARG;
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end()) return DSMC_ERR_ARG;
+    ServeReq &q = it->second;
+    if (q.state == REQ_DONE) return DSMC_ERR_ARG;
+    q.cancelled = 1;
+    if (q.state == REQ_QUEUED) {
+        for (auto qi = sv->queue.begin(); qi != sv->queue.end(); ++qi)
+            if (*qi == ticket) { sv->queue.erase(qi); break; }
+        finish(sv, ticket, q, DSMC_ERR_CANCELLED);
+    }
+    return DSMC_OK;
+}
+
+/* Forgets a completed request (its ticket becomes invalid). */
+extern "C" int dsmc_server_release(dsmc_server *sv, uint64_t ticket)
+{
+    if (!sv) return DSMC_ERR_ARG;
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end() || it->second.state != REQ_DONE) return DSMC_ERR_ARG;
+    sv->reqs.erase(it);
+    return DSMC_OK;
+}
+
+extern "C" int dsmc_server_get_stats(dsmc_server *sv, dsmc_server_stats *out)
+{
+    if (!sv || !out) return DSMC_ERR_ARG;
+    memset(out, 0, sizeof *out);
+    out->completed = sv->completed;
+    out->particle_steps = sv->steps_done;
+    out->queued = (int64_t)sv->queue.size();
+    if (sv->pool) {
+        Pool *P = sv->pool.get();
+        int64_t running = 0;
+        for (int g = 0; g < P->max_runs; g++) running += P->runs[g].used && !P->runs[g].done;
+        out->running = running;
+        out->particles_in_flight = P->inflight;
+        out->slots = P->max_slots;
+        out->rounds = P->rounds;
+        out->shape = P->shape;
+    }
+    return DSMC_OK;
+}
+
+extern "C" void dsmc_server_destroy(dsmc_server *sv) { delete sv; }

@@ -195,9 +195,12 @@ int dsmc_run_advanced(const dsmc_params *p, const dsmc_surface *s,
 
 /* Launch shape of the persistent particle kernel. */
 typedef struct dsmc_launch_shape {
-    int block;                /* threads per block of the flight kernel (256) */
+    int block;                /* threads per block of the flight kernel (64, 128, 192 or 256) */
     int blocks_per_cu;        /* persistent flight blocks per CU */
     int relaunch_batch;       /* idle lanes a wavefront gathers before pulling new particles */
+    int step_budget;          /* steps a particle takes per flight launch before yielding */
+    int service_inline;       /* physical runs: 1 = landings serviced inside the flight
+                                 kernel, 0 = by a separate service kernel per round */
 } dsmc_launch_shape;
 
 /* Staged runs: inputs uploaded once and kept resident in device memory, the run
@@ -230,25 +233,63 @@ int dsmc_build_fits(const dsmc_params *p, const dsmc_phase2_input *in,
                     int32_t *valid);
 
 /* ------------------------------------------------------------------ batch */
-/* Runs `nruns` independent runs in one persistent launch.  params[i],
- * surfaces[i] and results[i] describe run i.  `tune` selects the launch shape:
- * NULL = autotuned (cached per device and workload class). */
+/* Many independent physical runs (each its own body, species, surface
+ * temperatures, seed and particle count; one grid size and icosphere level for
+ * all) in one pool of device particle slots: their particles share persistent
+ * flight / service kernels whose rounds are replayed from one captured hipGraph
+ * until every run has finished.  `tune` selects the launch shape; NULL = the
+ * autotuned shape (tuned on a scaled copy of the batch of at least 4M particles
+ * or 1/16 of it, cached per device and workload class).  results[i].elapsed_ms
+ * = host time from the start of the batch until run i was seen finished. */
 int dsmc_run_batch(int nruns, const dsmc_params *params, const dsmc_surface *surfaces,
                    dsmc_result *results, const dsmc_launch_shape *tune);
+
+/* Staged form: inputs prepared and device memory allocated once (and the shape
+ * tuned), then launched any number of times.  per_particle: keep per-particle
+ * outputs (end_state, steps, fate, hops; 64 B per particle). */
+typedef struct dsmc_batch dsmc_batch;
+dsmc_batch *dsmc_batch_create(int nruns, const dsmc_params *params, const dsmc_surface *surfaces,
+                              int per_particle, const dsmc_launch_shape *tune, int *status);
+/* Runs every run to completion; results[i] (or results == NULL) receive run i. */
+int dsmc_batch_launch(dsmc_batch *b, dsmc_result *const *results);
+/* Host wall time of the last launch, each run's finish time, rounds, shape. */
+int dsmc_batch_timing(dsmc_batch *b, double *wall_ms, double *run_done_ms, int64_t *rounds,
+                      dsmc_launch_shape *shape);
+void dsmc_batch_destroy(dsmc_batch *b);
+
+/* Launch shape for runs like *p (the batch tuner on a one-run pool of at most
+ * 2^19 particles of it). */
 int dsmc_autotune(const dsmc_params *p, const dsmc_surface *s, dsmc_launch_shape *out);
 
 /* ------------------------------------------------------------------ serving */
-/* A continuous-batching server: requests are admitted while their HBM footprint
- * (particles x 96 B + grid) fits the budget, run together with whatever is
- * already on the device, and complete independently. */
+/* A continuous-batching server.  Requests (physical runs of one grid size and
+ * icosphere level) are prepared on the host at submission, queued in arrival
+ * order and admitted into one device pool as soon as particle slots (the HBM
+ * budget less max_inflight grid slots, at 268 B per particle, 332 B with
+ * per-particle outputs), a grid slot and table slots are free; they join the
+ * particles already in flight and complete independently.  A request's
+ * per-particle results do not depend on what else runs with it; its grid sums
+ * differ only by the order of floating-point atomic additions. */
 typedef struct dsmc_server dsmc_server;
-dsmc_server *dsmc_server_create(size_t hbm_budget_bytes, int max_inflight);
+typedef struct dsmc_server_stats {
+    int64_t completed, particle_steps;     /* requests completed, their particle-steps */
+    int64_t queued, running, particles_in_flight, slots, rounds;
+    dsmc_launch_shape shape;
+} dsmc_server_stats;
+dsmc_server *dsmc_server_create(size_t hbm_budget_bytes, int max_inflight, int per_particle);
+int dsmc_server_set_shape(dsmc_server *sv, const dsmc_launch_shape *shape);   /* while idle */
+/* r must stay valid until the request completes; r->elapsed_ms receives the time
+ * from submission to completion. */
 int dsmc_server_submit(dsmc_server *sv, const dsmc_params *p, const dsmc_surface *s,
                        dsmc_result *r, uint64_t *ticket);
+/* Admits what fits, keeps the device busy, completes finished requests; waits
+ * for at most one replay of the round graph.  Returns requests completed. */
+int dsmc_server_step(dsmc_server *sv);
 int dsmc_server_poll(dsmc_server *sv, uint64_t ticket);   /* 1 done, 0 pending, <0 error */
 int dsmc_server_wait(dsmc_server *sv, uint64_t ticket);
 int dsmc_server_cancel(dsmc_server *sv, uint64_t ticket);
-int dsmc_server_step(dsmc_server *sv);                     /* admit + advance, non-blocking */
+int dsmc_server_release(dsmc_server *sv, uint64_t ticket);  /* forget a completed request */
+int dsmc_server_get_stats(dsmc_server *sv, dsmc_server_stats *out);
 void dsmc_server_destroy(dsmc_server *sv);
 
 /* ------------------------------------------------------------------ drop-in */

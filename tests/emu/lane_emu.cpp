@@ -20,6 +20,81 @@ static void run_one(DsmcLane &L, const DsmcEnv &E, int64_t n, DsmcCounts &cnt)
     }
 }
 
+/* The same particle through the device kernels' record traffic: every launch
+ * starts from a lane filled with garbage (as registers are) and loads what the
+ * kernel loads.  SPLIT: start -> record; flight (rec_load_flight, at most
+ * `budget` steps, rec_store_flight); service (rec_load_all, lane_service,
+ * rec_store_all).  INLINE (physical): start -> record; launches of
+ * rec_load_phys, steps with the service inline, rec_store_phys on a pause. */
+static void garbage(DsmcLane &L) { memset(&L, 0xA5, sizeof L); }
+
+template <int MODE, int ADV>
+static void run_one_split(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t budget)
+{
+    DsmcRec R;
+    memset(&R, 0x5A, sizeof R);
+    DsmcLane L;
+    garbage(L);
+    lane_start<DsmcOpsHost, MODE, ADV>(L, E, n, cnt);
+    if (L.phase != LANE_FLIGHT) return;
+    rec_store_all<MODE>(L, R, 0);
+    DsmcCellTabs TB = {E.tq, E.hr, E.cs};
+    for (;;) {
+        DsmcLane F;
+        garbage(F);
+        rec_load_flight<MODE, ADV>(F, R);
+        const int64_t end = F.steps + budget;
+        bool paused = false;
+        while (F.phase == LANE_FLIGHT) {
+            lane_step<DsmcOpsHost, MODE, ADV>(F, E, TB);
+            if (F.phase == LANE_FLIGHT && F.steps >= end) { paused = true; break; }
+        }
+        lane_flush<DsmcOpsHost>(F, E);
+        rec_store_flight<MODE, ADV>(F, R);
+        if (paused) continue;
+        DsmcLane S;
+        garbage(S);
+        rec_load_all<MODE>(S, R);
+        lane_service<DsmcOpsHost, MODE, ADV>(S, E, cnt);
+        if (S.phase == LANE_DONE) return;
+        rec_store_all<MODE>(S, R, 0);
+    }
+}
+
+static void run_one_inline(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t budget)
+{
+    DsmcRec R;
+    memset(&R, 0x5A, sizeof R);
+    DsmcLane L;
+    garbage(L);
+    lane_start<DsmcOpsHost, 1, 0>(L, E, n, cnt);
+    if (L.phase != LANE_FLIGHT) return;
+    rec_store_all<1>(L, R, 0);
+    DsmcCellTabs TB = {E.tq, E.hr, E.cs};
+    for (;;) {
+        DsmcLane F;
+        garbage(F);
+        rec_load_phys(F, R);
+        const int64_t end = F.steps + budget;
+        for (;;) {
+            bool paused = false;
+            lane_step<DsmcOpsHost, 1, 0>(F, E, TB);
+            if (F.phase == LANE_FLIGHT && F.steps >= end) paused = true;
+            if (F.phase == LANE_LANDED || F.phase == LANE_ENDED) {
+                DsmcLane T = F;
+                lane_service<DsmcOpsHost, 1, 0>(T, E, cnt);
+                F = T;
+                if (F.phase == LANE_DONE) return;
+            }
+            if (paused) {
+                lane_flush<DsmcOpsHost>(F, E);
+                rec_store_phys(F, R);
+                break;
+            }
+        }
+    }
+}
+
 /* Advanced (collisional) compat run with fits supplied by the caller. */
 extern "C" int emu_advanced(const dsmc_params *p, const dsmc_surface *s, const double *copy_atm,
                             int64_t atm_total, const int32_t *count_num, const double *coeffs,
@@ -59,8 +134,13 @@ extern "C" int emu_advanced(const dsmc_params *p, const dsmc_surface *s, const d
     return 0;
 }
 
-extern "C" int emu_ballistic(const dsmc_params *p, const dsmc_surface *s, dsmc_result *r)
+/* path 0: one particle at a time straight through the state machine; 1: the
+ * split flight / service kernels' record traffic; 2: the physical flight kernel
+ * with the service inline.  `budget`: steps per flight launch (paths 1, 2). */
+extern "C" int emu_ballistic_path(const dsmc_params *p, const dsmc_surface *s, dsmc_result *r, int path,
+                                  int64_t budget)
 {
+    if (path < 0 || path > 2 || (path == 2 && p->mode != DSMC_MODE_PHYSICAL) || (path && budget < 1)) return -1;
     DsmcEnv E;
     dsmc::HostTables T;
     int rc = dsmc::build_env(p, s, 0, &E, &T);
@@ -75,6 +155,7 @@ extern "C" int emu_ballistic(const dsmc_params *p, const dsmc_surface *s, dsmc_r
     E.tq = T.tq.data(); E.hr = T.hr.data(); E.cs = T.cs.data();
     E.cdf = T.cdf.data(); E.face_T = s->face_T; E.vert = s->vert; E.faces = s->faces;
     E.enorm = T.enorm.data();
+    E.fnorm = T.fnorm.data();
     E.density = den.data(); E.horizon = hor.data();
     E.vmom = r->vmom;
     if (r->vmom) memset(r->vmom, 0, cells * 3 * sizeof(double));
@@ -90,7 +171,10 @@ extern "C" int emu_ballistic(const dsmc_params *p, const dsmc_surface *s, dsmc_r
     for (int64_t n = 0; n < p->nparticles; n++) {
         DsmcLane L;
         memset(&L, 0, sizeof L);
-        if (p->mode == DSMC_MODE_COMPAT) run_one<0, 0>(L, E, n, cnt);
+        if (path == 1 && p->mode == DSMC_MODE_COMPAT) run_one_split<0, 0>(E, n, cnt, budget);
+        else if (path == 1) run_one_split<1, 0>(E, n, cnt, budget);
+        else if (path == 2) run_one_inline(E, n, cnt, budget);
+        else if (p->mode == DSMC_MODE_COMPAT) run_one<0, 0>(L, E, n, cnt);
         else run_one<1, 0>(L, E, n, cnt);
     }
     if (r->density) memcpy(r->density, den.data(), cells * sizeof(double));
@@ -101,4 +185,9 @@ extern "C" int emu_ballistic(const dsmc_params *p, const dsmc_surface *s, dsmc_r
     r->n_depleted = cnt.depleted; r->n_capped = cnt.capped; r->n_collisions = cnt.coll;
     r->total_steps = cnt.steps;
     return 0;
+}
+
+extern "C" int emu_ballistic(const dsmc_params *p, const dsmc_surface *s, dsmc_result *r)
+{
+    return emu_ballistic_path(p, s, r, 0, 0);
 }

@@ -1,0 +1,191 @@
+"""Multi-run pools on the GPU: the batch API and the serving loop.
+
+Each run of a batch, and each served request, must give the same results as the
+same run alone: per-particle end states, step counts, fates and hop counts bit
+for bit, integer deposit counts, horizon (sums of unit weights) and fate
+counters exactly, and velocity moments to the summation-order tolerance
+|delta| <= 1e-12 * count * 5e3 m/s + 1e-9 (every term is one particle velocity
+below 5 km/s; only the order of the FP64 atomic additions differs).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from dsmc_suite_amd import capi
+from dsmc_suite_amd import workloads as W
+from dsmc_suite_amd.capi import Buffers
+
+from tests.helpers import oracle
+
+pytestmark = pytest.mark.gpu
+
+LEVELS, HD = 3, 40
+COUNTERS = ("n_escaped", "n_frozen", "n_photolysed", "n_depleted", "n_capped", "n_collisions", "total_steps")
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return capi.load()
+
+
+def specs():
+    return [W.RunSpec(W.BODIES[0], 3000, 0.3, 11), W.RunSpec(W.BODIES[1], 700, 2.0, 12),
+            W.RunSpec(W.BODIES[2], 2500, 4.1, 13), W.RunSpec(W.BODIES[3], 1500, 5.5, 14),
+            W.RunSpec(W.BODIES[0], 257, 1.0, 15)]
+
+
+def alone(lib, p, surf):
+    b = Buffers(p, 5, per_particle=True, moments=True)
+    st = C.c_int(0)
+    run = lib.dsmc_run_create(C.byref(p), C.byref(surf.surface), C.byref(b.result), C.byref(st))
+    assert run, st.value
+    assert lib.dsmc_run_launch(run, None) == 0
+    assert lib.dsmc_run_sync(run) == 0
+    assert lib.dsmc_run_collect(run, C.byref(b.result)) == 0
+    lib.dsmc_run_destroy(run)
+    return b, {k: int(getattr(b.result, k)) for k in COUNTERS}
+
+
+def assert_same_run(a: Buffers, ca: dict, b: Buffers, cb: dict, what: str):
+    assert ca == cb, what
+    for f in ("end_state", "steps", "fate", "hops", "vcount", "density", "horizon"):
+        x, y = getattr(a, f), getattr(b, f)
+        assert np.array_equal(x, y), f"{what}: {f} differs at {np.flatnonzero(x != y)[:5]}"
+    cnt = np.repeat(a.vcount.astype(np.float64), 3)
+    tol = 1e-12 * cnt * 5e3 + 1e-9
+    d = np.abs(a.vmom - b.vmom)
+    assert np.all(d <= tol), f"{what}: vmom max excess {np.max(d - tol)}"
+
+
+def test_batch_runs_equal_single_runs(lib):
+    sp = specs()
+    params, surfs = W.build_runs(lib, sp, levels=LEVELS, hd=HD)
+    bufs = [Buffers(p, 5, per_particle=True, moments=True) for p in params]
+    pa, sa = W.as_arrays(params, surfs)
+    res = (capi.Result * len(bufs))(*[b.result for b in bufs])
+    shape = capi.LaunchShape(256, 4, 16, 1024, 1)
+    print(f"batch of {len(sp)} runs, counts {[s.n for s in sp]}", flush=True)
+    assert lib.dsmc_run_batch(len(sp), pa, sa, res, C.byref(shape)) == 0
+    for i, (p, s) in enumerate(zip(params, surfs)):
+        b1, c1 = alone(lib, p, s)
+        cb = {k: int(getattr(res[i], k)) for k in COUNTERS}
+        assert cb["total_steps"] > 0
+        assert sum(cb[k] for k in COUNTERS[:5]) == p.nparticles
+        assert_same_run(b1, c1, bufs[i], cb, f"run {i} ({sp[i].body.name}, n={sp[i].n})")
+
+
+def test_batch_run_matches_oracle(lib):
+    """One run of a batch against the CPU oracle (device-math build)."""
+    sp = specs()[:2]
+    params, surfs = W.build_runs(lib, sp, levels=LEVELS, hd=HD)
+    bufs = [Buffers(p, 5, per_particle=True, moments=True) for p in params]
+    pa, sa = W.as_arrays(params, surfs)
+    res = (capi.Result * 2)(*[b.result for b in bufs])
+    assert lib.dsmc_run_batch(2, pa, sa, res, None) == 0     # autotuned shape
+    orc = oracle(devmath=True)
+    p = params[1]
+    ob = Buffers(p, 5, per_particle=True, moments=True)
+    assert orc.cpu_ref_ballistic(C.byref(p), C.byref(surfs[1].surface), C.byref(ob.result), 4, 0) == 0
+    for f in ("end_state", "steps", "fate", "hops", "vcount"):
+        assert np.array_equal(getattr(ob, f), getattr(bufs[1], f)), f
+    assert int(res[1].total_steps) == int(ob.result.total_steps)
+
+
+def test_staged_batch_relaunch(lib):
+    """A staged batch launched twice gives the same results; timing is reported."""
+    sp = specs()[:3]
+    params, surfs = W.build_runs(lib, sp, levels=LEVELS, hd=HD)
+    pa, sa = W.as_arrays(params, surfs)
+    st = C.c_int(0)
+    b = lib.dsmc_batch_create(3, pa, sa, 0, None, C.byref(st))
+    assert b, st.value
+    outs = []
+    for _ in range(2):
+        bufs = [Buffers(p, 5, per_particle=False, moments=True) for p in params]
+        arr = (C.POINTER(capi.Result) * 3)(*[C.pointer(x.result) for x in bufs])
+        assert lib.dsmc_batch_launch(b, arr) == 0
+        outs.append(bufs)
+    wall, rounds = C.c_double(), C.c_int64()
+    done = np.zeros(3)
+    shape = capi.LaunchShape()
+    assert lib.dsmc_batch_timing(b, C.byref(wall), capi.ptr(done, C.c_double), C.byref(rounds), C.byref(shape)) == 0
+    lib.dsmc_batch_destroy(b)
+    assert wall.value > 0 and rounds.value > 0 and np.all(done > 0) and np.all(done <= wall.value + 1e-6)
+    assert shape.block in (64, 128, 192, 256)
+    for x, y in zip(*outs):
+        assert np.array_equal(x.vcount, y.vcount) and np.array_equal(x.horizon, y.horizon)
+        assert x.result.total_steps == y.result.total_steps
+
+
+def test_server_alone_equals_mix(lib):
+    """Continuous batching: run A alone, then A admitted while other runs fly."""
+    sp = specs()
+    params, surfs = W.build_runs(lib, sp, levels=LEVELS, hd=HD)
+    sv = lib.dsmc_server_create(2 << 30, 8, 1)
+    assert sv
+    try:
+        t = C.c_uint64()
+        ra = Buffers(params[0], 5, per_particle=True, moments=True)
+        assert lib.dsmc_server_submit(sv, C.byref(params[0]), C.byref(surfs[0].surface), C.byref(ra.result),
+                                      C.byref(t)) == 0
+        assert lib.dsmc_server_wait(sv, t.value) == 0
+        ca = {k: int(getattr(ra.result, k)) for k in COUNTERS}
+        assert ra.result.elapsed_ms > 0
+
+        mix = [Buffers(p, 5, per_particle=True, moments=True) for p in params]
+        tickets = {}
+        order = [1, 3, 0, 2, 4]
+        saw_overlap = False
+        for j, i in enumerate(order):
+            tk = C.c_uint64()
+            assert lib.dsmc_server_submit(sv, C.byref(params[i]), C.byref(surfs[i].surface),
+                                          C.byref(mix[i].result), C.byref(tk)) == 0
+            tickets[i] = tk.value
+            assert lib.dsmc_server_step(sv) >= 0
+            stt = capi.ServerStats()
+            lib.dsmc_server_get_stats(sv, C.byref(stt))
+            saw_overlap |= stt.running >= 2
+        for i in order:
+            assert lib.dsmc_server_wait(sv, tickets[i]) == 0
+        assert saw_overlap, "requests never overlapped on the device"
+        cm = {k: int(getattr(mix[0].result, k)) for k in COUNTERS}
+        assert_same_run(ra, ca, mix[0], cm, "request A alone vs in the mix")
+        for i in order[1:]:
+            b1, c1 = alone(lib, params[i], surfs[i])
+            ci = {k: int(getattr(mix[i].result, k)) for k in COUNTERS}
+            assert_same_run(b1, c1, mix[i], ci, f"request {i}")
+    finally:
+        lib.dsmc_server_destroy(sv)
+
+
+def test_server_budget_and_cancel(lib):
+    sp = specs()
+    params, surfs = W.build_runs(lib, sp, levels=LEVELS, hd=HD)
+    sv = lib.dsmc_server_create(200 << 20, 4, 0)
+    assert sv
+    try:
+        big = params[0].copy()
+        big.nparticles = 10_000_000
+        r = Buffers(big, 5, per_particle=False, moments=True)
+        t = C.c_uint64()
+        assert lib.dsmc_server_submit(sv, C.byref(big), C.byref(surfs[0].surface), C.byref(r.result),
+                                      C.byref(t)) == -5     # DSMC_ERR_BUDGET
+        ra = Buffers(params[0], 5, per_particle=False, moments=True)
+        rb = Buffers(params[2], 5, per_particle=False, moments=True)
+        ta, tb = C.c_uint64(), C.c_uint64()
+        assert lib.dsmc_server_submit(sv, C.byref(params[0]), C.byref(surfs[0].surface), C.byref(ra.result),
+                                      C.byref(ta)) == 0
+        assert lib.dsmc_server_submit(sv, C.byref(params[2]), C.byref(surfs[2].surface), C.byref(rb.result),
+                                      C.byref(tb)) == 0
+        assert lib.dsmc_server_cancel(sv, tb.value) == 0
+        assert lib.dsmc_server_poll(sv, tb.value) == -6     # DSMC_ERR_CANCELLED
+        assert lib.dsmc_server_wait(sv, ta.value) == 0
+        assert ra.result.total_steps > 0
+        assert lib.dsmc_server_release(sv, ta.value) == 0
+        assert lib.dsmc_server_poll(sv, ta.value) == -1
+    finally:
+        lib.dsmc_server_destroy(sv)

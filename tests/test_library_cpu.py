@@ -149,6 +149,55 @@ def test_lane_state_machine_matches_oracle_physical(lib, emu):
     assert np.allclose(got.density, want.density, rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("path,budget", [(1, 37), (2, 37), (1, 4096), (2, 1024)])
+def test_record_paths_match_oracle(lib, emu, path, budget):
+    """The kernels' record traffic (split flight/service kernels, or the physical
+    flight kernel with the service inline), each launch starting from garbage
+    registers, with short step budgets (many pauses): bit-identical to the oracle
+    for a short-hop body (many landings) and for compat runs."""
+    from dsmc_suite_amd import workloads as W
+    dev = oracle(devmath=True)
+    b = W.BODIES[3]
+    surf = W.icosphere(lib, 3, b, 2.0)
+    p = W.body_params(lib, b, 60, 2.0, 77, hd=40)
+    want = run_oracle(dev, p, surf, nthreads=4, moments=True)
+    got = Buffers(p, 5, moments=True)
+    emu.emu_ballistic_path.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int64]
+    assert emu.emu_ballistic_path(C.byref(p), C.byref(surf.surface), C.byref(got.result), path, budget) == 0
+    for k in ("steps", "end_state", "fate", "hops", "vcount", "horizon"):
+        assert np.array_equal(getattr(got, k), getattr(want, k)), k
+    assert got.counters() == want.counters()
+    assert want.hops.mean() > 20
+    if path == 1:
+        T, ftd, ffd = read_T(GOLDEN / "T_thermal.dat")
+        p = compat_params(ftd, ffd, 80, 5)
+        s = SurfaceArrays(T, ftd, ffd)
+        want = run_oracle(dev, p, s, nthreads=4)
+        got = Buffers(p, 5)
+        assert emu.emu_ballistic_path(C.byref(p), C.byref(s.surface), C.byref(got.result), 1, budget) == 0
+        for k in ("ttime", "steps", "end_state", "fate", "hops"):
+            assert np.array_equal(getattr(got, k), getattr(want, k)), k
+
+
+@pytest.mark.parametrize("body", range(4))
+def test_lane_state_machine_matches_oracle_bodies(lib, emu, body):
+    """Every body of the multi-run workloads, on the lane state machine: short
+    hops and many landings (the guessed-face test and the cell walks from the
+    landing cell must give what the oracle's full searches give)."""
+    from dsmc_suite_amd import workloads as W
+    b = W.BODIES[body]
+    dev = oracle(devmath=True)
+    surf = W.icosphere(lib, 3, b, 0.7 + body)
+    p = W.body_params(lib, b, 120, 0.7 + body, 40 + body, hd=40)
+    want = run_oracle(dev, p, surf, nthreads=4, moments=True)
+    got = Buffers(p, 5, moments=True)
+    assert emu.emu_ballistic(C.byref(p), C.byref(surf.surface), C.byref(got.result)) == 0
+    for k in ("steps", "end_state", "fate", "hops", "vcount", "horizon"):
+        assert np.array_equal(getattr(got, k), getattr(want, k)), k
+    assert got.counters() == want.counters()
+    assert want.hops.sum() > len(want.hops)          # relaunches happened
+
+
 def test_device_math_accuracy_against_glibc():
     """dsmc_math.h against glibc: at most 1 ulp for sin/cos/exp/log and 3 ulp for
     asin/atan2 on the argument ranges the kernels use."""
