@@ -110,7 +110,7 @@ def plain_run(lib, surf, n, args, shape) -> dict:
             "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                          "flops_per_particle_step": FLOPS_PER_STEP,
-                         "kernel": "phys_kernel<false>" if (shape is None or shape.service_inline) else
+                         "kernel": "phys_kernel<false>" if (shape is not None and shape.service_inline) else
                          "flight_kernel<1,0>", "kernel_avg_ms_per_run": favg * 1e3}}
 
 
@@ -247,7 +247,7 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--relaunch-batch", type=int, default=0)
     ap.add_argument("--step-budget", type=int, default=0)
-    ap.add_argument("--service-inline", type=int, default=1)
+    ap.add_argument("--service-inline", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=60000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-moments", action="store_true", help="density only (no velocity moments)")
@@ -269,7 +269,7 @@ def main():
     lib = capi.load()
     surf = icosphere(lib, 3)
     shape = None
-    if args.block or args.blocks_per_cu or args.relaunch_batch or args.step_budget or args.service_inline != 1:
+    if args.block or args.blocks_per_cu or args.relaunch_batch or args.step_budget or args.service_inline:
         shape = capi.LaunchShape(args.block or 256, args.blocks_per_cu or 4, args.relaunch_batch or 16,
                                  args.step_budget or 1024, args.service_inline)
     main_run = plain_run(lib, surf, args.particles, args, shape)

@@ -34,7 +34,7 @@ struct RunDev {
     /* advanced */
     DevBuf atm, count_num, coeffs, pmin, pmax, pvalid, speeds;
     /* particle records and slot queues */
-    DevBuf rec, flyq0, flyq1, svcq, qctl, counters, rec_total;
+    DevBuf rec, flyq0, flyq1, svcq, qctl, counters, rec_total, env_d;
     unsigned int *host_ctl = nullptr;     /* pinned readback of qctl */
     int rounds = 0;
     double flight_ms = 0.0;

@@ -41,9 +41,9 @@ hipError_t dsmc_launch_pool_flight(const DsmcPoolDev *P, DsmcQueues Q, int p, in
                                    int refill_batch, int64_t step_budget, hipStream_t st);
 hipError_t dsmc_launch_pool_service(const DsmcPoolDev *P, DsmcQueues Q, int p, int grid, hipStream_t st);
 /* physical runs: flight with the landing service inline, then the queue reset;
- * one run (E, P == NULL) or a pool (P) */
-hipError_t dsmc_launch_phys(const DsmcEnv *E, const DsmcPoolDev *P, DsmcRec *rec, DsmcQueues Q, int p, int grid,
-                            int block, int svc_batch, int64_t step_budget, hipStream_t st);
+ * one run (E on the host, Ed its copy in device memory, P == NULL) or a pool (P) */
+hipError_t dsmc_launch_phys(const DsmcEnv *E, const DsmcEnv *Ed, const DsmcPoolDev *P, DsmcRec *rec, DsmcQueues Q,
+                            int p, int grid, int block, int svc_batch, int64_t step_budget, hipStream_t st);
 /* n zeros of 8 bytes (graph-safe clear) */
 hipError_t dsmc_launch_fill0(void *p, size_t n8, hipStream_t st);
 hipError_t dsmc_launch_verlet(double *state, int n, int nsteps, double GM, double R,

@@ -622,13 +622,17 @@ DSMC_HD int locate_face(const DsmcEnv &E, double dx, double dy, double dz)
         double m = face_margin(E, f, dx, dy, dz);
         if (m > bm) { bm = m; best = f; }
     }
+    /* level l starts at face 20 (4^l - 1) / 3 of the all-levels arrays */
+    int off = 0, nl = 20;
 #pragma unroll 1
     for (int l = 1; l <= E.levels; l++) {
+        off += nl;
+        nl *= 4;
         int base = 4 * best, pick = base;
-        double pm = face_margin(E, E.level_off[l] + base, dx, dy, dz);
+        double pm = face_margin(E, off + base, dx, dy, dz);
 #pragma unroll 1
         for (int j = 1; j < 4; j++) {
-            double m = face_margin(E, E.level_off[l] + base + j, dx, dy, dz);
+            double m = face_margin(E, off + base + j, dx, dy, dz);
             if (m > pm) { pm = m; pick = base + j; }
         }
         best = pick;

@@ -186,6 +186,8 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
     if ((rc = rd->qctl.zeros(sizeof(unsigned int) * 16, st))) return rc;
     if (!rd->host_ctl && hipHostMalloc((void **)&rd->host_ctl, sizeof(unsigned int) * 16, 0) != hipSuccess)
         return DSMC_ERR_NOMEM;
+    /* room for the environment in device memory (written by launch_run) */
+    if ((rc = rd->env_d.alloc(sizeof(DsmcEnv)))) return rc;
     return DSMC_OK;
 }
 
@@ -212,6 +214,8 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
     Q.ctl = (unsigned int *)rd->qctl.ptr;
     DsmcRec *rec = (DsmcRec *)rd->rec.ptr;
     HIP_OK(hipMemsetAsync(Q.ctl, 0, sizeof(unsigned int) * 16, st));
+    /* the run's environment as the kernels' service code reads it from device memory */
+    HIP_OK(hipMemcpyAsync(rd->env_d.ptr, &rd->E, sizeof(DsmcEnv), hipMemcpyHostToDevice, st));
     rd->rounds = 0;
     rd->flight_ms = 0.0;
     if (n == 0) return DSMC_OK;
@@ -238,8 +242,8 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
                 HIP_OK(hipEventRecord(a, st));
             }
             if (inline_svc) {
-                HIP_OK(dsmc_launch_phys(&rd->E, nullptr, rec, Q, p, flight_grid, sh.block, sh.relaunch_batch,
-                                        sh_budget, st));
+                HIP_OK(dsmc_launch_phys(&rd->E, (const DsmcEnv *)rd->env_d.ptr, nullptr, rec, Q, p, flight_grid,
+                                        sh.block, sh.relaunch_batch, sh_budget, st));
                 if (timed) HIP_OK(hipEventRecord(ev.back(), st));
             } else {
                 HIP_OK(dsmc_launch_flight(&rd->E, rec, Q, p, flight_grid, sh.block, sh.relaunch_batch, sh_budget,
@@ -275,7 +279,7 @@ dsmc_launch_shape default_shape(const DsmcEnv &E)
     s.blocks_per_cu = 4;
     s.relaunch_batch = E.mode == 0 ? 8 : 16;
     s.step_budget = E.mode == 0 ? 4096 : 1024;
-    s.service_inline = E.mode == DSMC_MODE_PHYSICAL ? 1 : 0;
+    s.service_inline = 0;     /* long-hop single runs: the lean flight kernel (autotuning may pick 1) */
     return s;
 }
 

@@ -126,6 +126,32 @@ __device__ static inline void wave_append(bool pred, int32_t slot, int32_t *q, u
     if (pred) q[base + __popcll(m & lanemask_lt())] = slot;
 }
 
+/* Fate / step counters of finished particles, aggregated over the lanes of the
+ * wavefront that share a counter block (one run): one atomic per counter and
+ * run instead of one per lane.  Called by every lane of the wavefront. */
+__device__ static inline void counts_flush_wave(DsmcCounts &cnt, unsigned long long *ctr, bool active)
+{
+    active = active && ctr != nullptr;
+    for (;;) {
+        uint64_t act = __ballot(active);
+        if (act == 0) break;
+        const int leader = __ffsll((long long)act) - 1;
+        unsigned long long *lead = (unsigned long long *)__shfl((long long)ctr, leader);
+        const bool mine = active && ctr == lead;
+        unsigned long long v[8] = {cnt.esc, cnt.frozen, cnt.photo, cnt.depleted, cnt.capped, cnt.coll, cnt.steps,
+                                   cnt.done};
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            unsigned long long x = mine ? v[i] : 0ull;
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) x += (unsigned long long)__shfl_xor((long long)x, o);
+            if ((int)__lane_id() == leader && x) atomicAdd(&lead[i], x);
+        }
+        active = active && !mine;
+    }
+    cnt.esc = cnt.frozen = cnt.photo = cnt.depleted = cnt.capped = cnt.coll = cnt.steps = cnt.done = 0;
+}
+
 /* ------------------------------------------------------------------ start */
 /* Starts particles [0, n) of one run in slots slot0..slot0+n-1. */
 template <int MODE, int ADV>
@@ -304,10 +330,13 @@ pool_flight_kernel(DsmcPoolDev P, DsmcQueues Q, int p, int refill_batch, int64_t
 }
 
 /* The physical landing service, out of line: the flight kernel's registers stay
- * those of the Verlet step; the lane state travels through the call frame. */
+ * those of the Verlet step; the lane state travels through the call frame.  The
+ * run's environment is copied from device memory once, at the top (its fields
+ * load together instead of one dependent load per use). */
 template <class Ops>
-__device__ __attribute__((noinline)) void phys_service(DsmcLane &T, const DsmcEnv &E, DsmcCounts &cnt)
+__device__ __attribute__((noinline)) void phys_service(DsmcLane &T, const DsmcEnv *Ep, DsmcCounts &cnt)
 {
+    const DsmcEnv E = *Ep;
     lane_service<Ops, 1, 0>(T, E, cnt);
 }
 
@@ -322,7 +351,8 @@ __device__ __attribute__((noinline)) void phys_service(DsmcLane &T, const DsmcEn
  * deposits go through the LDS slot table. */
 template <bool POOL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
-phys_kernel(DsmcEnv E0, DsmcPoolDev P, DsmcRec *rec, DsmcQueues Q, int p, int svc_batch, int64_t step_budget)
+phys_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcPoolDev P, DsmcRec *rec, DsmcQueues Q, int p, int svc_batch,
+            int64_t step_budget)
 {
     extern __shared__ double lds_tabs[];
     const int ftd = POOL ? P.ftd : E0.ftd, ffd = POOL ? P.ffd : E0.ffd, hd = POOL ? P.hd : E0.hd;
@@ -389,18 +419,17 @@ phys_kernel(DsmcEnv E0, DsmcPoolDev P, DsmcRec *rec, DsmcQueues Q, int p, int sv
             if ((ended && __popcll(waiting) >= svc_batch) || waiting == __ballot(true)) break;
         }
         /* landings and ended particles: service inline */
-        if (L.phase == LANE_LANDED || L.phase == LANE_ENDED) {
-            const DsmcEnv &E = POOL ? P.envs[run] : E0;
-            DsmcCounts cnt = {0, 0, 0, 0, 0, 0, 0, 0};
+        DsmcCounts cnt = {0, 0, 0, 0, 0, 0, 0, 0};
+        const bool svc = L.phase == LANE_LANDED || L.phase == LANE_ENDED;
+        if (svc) {
             DsmcLane T = L;
-            if (POOL) phys_service<DsmcOpsDev>(T, E, cnt);
-            else phys_service<DsmcOpsLds>(T, E, cnt);
+            if (POOL) phys_service<DsmcOpsDev>(T, P.envs + run, cnt);
+            else phys_service<DsmcOpsLds>(T, Ed, cnt);
             L = T;
-            if (L.phase == LANE_DONE) {
-                counts_flush<DsmcOpsDev>(cnt, E);
-                L.phase = LANE_IDLE;
-            }
         }
+        const bool fin = svc && L.phase == LANE_DONE;
+        if (__ballot(fin)) counts_flush_wave(cnt, fin ? (POOL ? P.envs[run].counters : E0.counters) : nullptr, fin);
+        if (fin) L.phase = LANE_IDLE;
         /* budget-paused particles go to the next launch */
         bool paused = L.phase == LANE_IDLE + 8;
         if (paused) {
@@ -428,20 +457,23 @@ service_kernel(DsmcEnv E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int 
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t b = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < (int64_t)n; b += stride) {
         int64_t i = b + __lane_id();
-        bool fly = false;
+        bool fly = false, fin = false;
         int32_t slot = 0;
+        unsigned long long *ctr = nullptr;
+        DsmcCounts cnt = {0, 0, 0, 0, 0, 0, 0, 0};
         if (i < n) {
             slot = Q.svc[i];
             int run = POOL ? rec[slot].run : 0;
-            const DsmcEnv &E = POOL ? envs[run] : E0;
+            const DsmcEnv E = POOL ? envs[run] : E0;      /* one wide load per lane in a pool */
             DsmcLane L;
             rec_load_all<MODE>(L, rec[slot]);
-            DsmcCounts cnt = {0, 0, 0, 0, 0, 0, 0, 0};
             lane_service<DsmcOpsDev, MODE, ADV>(L, E, cnt);
-            counts_flush<DsmcOpsDev>(cnt, E);
             fly = L.phase == LANE_FLIGHT;
+            fin = L.phase == LANE_DONE;
+            ctr = E.counters;
             if (fly) rec_store_all<MODE>(L, rec[slot], run);
         }
+        if (__ballot(fin)) counts_flush_wave(cnt, ctr, fin);
         wave_append(fly, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
     }
 }
@@ -568,8 +600,9 @@ extern "C" hipError_t dsmc_launch_pool_flight(const DsmcPoolDev *P, DsmcQueues Q
     return hipGetLastError();
 }
 
-extern "C" hipError_t dsmc_launch_phys(const DsmcEnv *E, const DsmcPoolDev *P, DsmcRec *rec, DsmcQueues Q, int p,
-                                       int grid, int block, int svc_batch, int64_t step_budget, hipStream_t st)
+extern "C" hipError_t dsmc_launch_phys(const DsmcEnv *E, const DsmcEnv *Ed, const DsmcPoolDev *P, DsmcRec *rec,
+                                       DsmcQueues Q, int p, int grid, int block, int svc_batch, int64_t step_budget,
+                                       hipStream_t st)
 {
     if (!block_ok(block)) return hipErrorInvalidValue;
     if (P) {
@@ -577,15 +610,15 @@ extern "C" hipError_t dsmc_launch_phys(const DsmcEnv *E, const DsmcPoolDev *P, D
         const size_t lds = sizeof(double) * (size_t)((P->ftd + 1) + 2 * (P->ffd + 1) + P->nhr);
         DsmcEnv none;
         memset(&none, 0, sizeof none);
-        hipLaunchKernelGGL((phys_kernel<true>), dim3(grid), dim3(block), lds, st, none, *P, rec, Q, p, svc_batch,
-                           step_budget);
+        hipLaunchKernelGGL((phys_kernel<true>), dim3(grid), dim3(block), lds, st, none, (const DsmcEnv *)nullptr, *P,
+                           rec, Q, p, svc_batch, step_budget);
     } else {
-        if (E->mode != 1 || E->advanced) return hipErrorInvalidValue;
+        if (E->mode != 1 || E->advanced || !Ed) return hipErrorInvalidValue;
         const size_t lds = sizeof(double) * (size_t)((E->ftd + 1) + 2 * (E->ffd + 1) + (E->hd + 2));
         DsmcPoolDev none;
         memset(&none, 0, sizeof none);
-        hipLaunchKernelGGL((phys_kernel<false>), dim3(grid), dim3(block), lds, st, *E, none, rec, Q, p, svc_batch,
-                           step_budget);
+        hipLaunchKernelGGL((phys_kernel<false>), dim3(grid), dim3(block), lds, st, *E, Ed, none, rec, Q, p,
+                           svc_batch, step_budget);
     }
     hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, Q.ctl, p, Q.ctl + 8);
     return hipGetLastError();

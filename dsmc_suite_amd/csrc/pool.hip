@@ -136,8 +136,8 @@ static int launch_rounds(Pool *P, hipStream_t s)
     for (int r = 0; r < P->graph_rounds; r++) {
         int p = r & 1;
         if (P->shape.service_inline) {
-            HIP_OK(dsmc_launch_phys(nullptr, &D, D.rec, Q, p, fgrid, P->shape.block, P->shape.relaunch_batch,
-                                    P->shape.step_budget, s));
+            HIP_OK(dsmc_launch_phys(nullptr, nullptr, &D, D.rec, Q, p, fgrid, P->shape.block,
+                                    P->shape.relaunch_batch, P->shape.step_budget, s));
         } else {
             HIP_OK(dsmc_launch_pool_flight(&D, Q, p, fgrid, P->shape.block, P->shape.relaunch_batch,
                                            P->shape.step_budget, s));
