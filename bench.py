@@ -248,7 +248,7 @@ def main():
     ap.add_argument("--relaunch-batch", type=int, default=0)
     ap.add_argument("--step-budget", type=int, default=0)
     ap.add_argument("--service-inline", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=60000)
+    ap.add_argument("--cpu-sample", type=int, default=1500000, help="particles of the CPU baseline sample (~15 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-moments", action="store_true", help="density only (no velocity moments)")
     ap.add_argument("--no-batch", action="store_true")

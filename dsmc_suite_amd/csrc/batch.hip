@@ -96,7 +96,7 @@ static int coordinate_descent(dsmc_launch_shape *best, Trial trial, double *best
         if (sc > bs * 1.01) { bs = sc; *best = c; }
         return DSMC_OK;
     };
-    const int waves_per_cu[] = {8, 12, 16, 24, 32};
+    const int waves_per_cu[] = {8, 12, 16, 20, 24};
     dsmc_launch_shape base = *best;
     for (int w : waves_per_cu) {
         dsmc_launch_shape c = base;
@@ -113,7 +113,7 @@ static int coordinate_descent(dsmc_launch_shape *best, Trial trial, double *best
         if ((rc = try_shape(c))) return rc;
     }
     base = *best;
-    for (int r : {4, 8, 16, 32}) {
+    for (int r : {16, 32, 48, 64}) {
         if (r == base.relaunch_batch) continue;
         dsmc_launch_shape c = base;
         c.relaunch_batch = r;

@@ -259,7 +259,7 @@ int pool_create(Pool *P, const PoolClass &c, int max_runs, int64_t max_slots, in
     dsmc_launch_shape sh;
     sh.block = 256;
     sh.blocks_per_cu = 4;
-    sh.relaunch_batch = 16;
+    sh.relaunch_batch = 48;     /* lanes serviced together with the landing service inline */
     sh.step_budget = 1024;
     sh.service_inline = 1;
     return pool_set_shape(P, shape ? *shape : sh);
