@@ -40,7 +40,7 @@ class Params(C.Structure):
         ("record_cap_cell", C.c_int), ("record_cap_total", C.c_int64),
         ("cross_section", C.c_double), ("fit_degree", C.c_int), ("max_per_cell", C.c_int),
         ("min_fit_samples", C.c_int), ("density_scale", C.c_double),
-        ("sublimation_scale", C.c_double),
+        ("sublimation_scale", C.c_double), ("vhist_bins", C.c_int), ("vhist_max", C.c_double),
     ]
 
     def copy(self) -> "Params":
@@ -61,6 +61,7 @@ class Result(C.Structure):
     _fields_ = [
         ("density", _dp), ("horizon", _dp), ("vmom", _dp), ("vcount", _i64p),
         ("ttime", _dp), ("end_state", _dp), ("steps", _i64p), ("fate", _i32p), ("hops", _i32p),
+        ("vhist", _i64p),
         ("vrec", _dp), ("vrec_td", _i32p), ("vrec_fd", _i32p), ("vrec_ht", _i32p),
         ("vrec_n", C.c_int64),
         ("n_escaped", C.c_int64), ("n_frozen", C.c_int64), ("n_photolysed", C.c_int64),
@@ -73,7 +74,7 @@ class Phase2Input(C.Structure):
     _fields_ = [
         ("copy_atm", _dp), ("atm_ftd", C.c_int), ("atm_ffd", C.c_int), ("atm_hd", C.c_int),
         ("vrec", _dp), ("vrec_td", _i32p), ("vrec_fd", _i32p), ("vrec_ht", _i32p),
-        ("nrec", C.c_int64),
+        ("nrec", C.c_int64), ("ndens", _dp), ("vhist", _i64p),
     ]
 
 
@@ -100,7 +101,7 @@ class Buffers:
     """Host arrays backing a Result; keeps them alive and exposes numpy views."""
 
     def __init__(self, p: Params, ttime_rows: int = 5, per_particle: bool = True,
-                 vrec: bool = False, moments: bool = False):
+                 vrec: bool = False, moments: bool = False, vhist: bool = False):
         n = int(p.nparticles)
         cells = p.hd * p.ftd * p.ffd
         self.density = np.zeros(cells)
@@ -112,6 +113,7 @@ class Buffers:
         self.steps = np.zeros(n, dtype=np.int64) if per_particle else None
         self.fate = np.zeros(n, dtype=np.int32) if per_particle else None
         self.hops = np.zeros(n, dtype=np.int32) if per_particle else None
+        self.vhist = np.zeros(cells * p.vhist_bins, dtype=np.int64) if vhist else None
         cap = int(p.record_cap_total) if vrec else 0
         self.vrec = np.zeros(cap * 3) if vrec else None
         self.vrec_td = np.zeros(cap, dtype=np.int32) if vrec else None
@@ -127,6 +129,7 @@ class Buffers:
         r.steps = ptr(self.steps, C.c_int64)
         r.fate = ptr(self.fate, C.c_int32)
         r.hops = ptr(self.hops, C.c_int32)
+        r.vhist = ptr(self.vhist, C.c_int64)
         r.vrec = ptr(self.vrec, C.c_double)
         r.vrec_td = ptr(self.vrec_td, C.c_int32)
         r.vrec_fd = ptr(self.vrec_fd, C.c_int32)
@@ -175,6 +178,9 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.dsmc_run_advanced.argtypes = [P, S, C.POINTER(Phase2Input), R]
     lib.dsmc_build_fits.argtypes = [P, C.POINTER(Phase2Input), _i32p, _dp, _dp, _dp, _i32p]
     lib.dsmc_run_batch.argtypes = [C.c_int, P, S, R, C.POINTER(LaunchShape)]
+    lib.dsmc_number_density.argtypes = [P, S, _i64p, _dp]
+    lib.dsmc_build_hist_fits.argtypes = [P, _i64p, _dp, _dp, _dp, _i32p]
+    lib.dsmc_run_collisional.argtypes = [P, P, S, R, R]
     lib.dsmc_autotune.argtypes = [P, S, C.POINTER(LaunchShape)]
     lib.dsmc_icosphere_sizes.argtypes = [C.c_int, _i32p, _i32p, _i32p]
     lib.dsmc_icosphere_build.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,

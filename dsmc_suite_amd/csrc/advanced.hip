@@ -182,7 +182,99 @@ fit_kernel(const int32_t *hist, const int32_t *start, const int32_t *bucket, con
     }
 }
 
+/* Physical collisional runs: partner-speed fits from phase-1 speed histograms,
+ * one lane per cell (cells [ht][td][fd]).  P at the occupied bins lo..hi is 50 x
+ * (steps in bins >= k) / (steps of the cell), fitted in the normalised speed
+ * x = (k - lo + 0.5) / (hi - lo + 1); valid when the cell has more than
+ * min_samples steps and more occupied-range bins than coefficients. */
+__global__ void hist_fit_kernel(const unsigned long long *vhist, int ncell, int bins, double dv, int degree,
+                                int min_samples, double *coeffs, double *pmin, double *pmax, int32_t *valid,
+                                int32_t *count_num)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const unsigned long long *h = vhist + (size_t)c * bins;
+    unsigned long long N = 0;
+    int lo = -1, hi = -1;
+    for (int b = 0; b < bins; b++)
+        if (h[b]) { N += h[b]; if (lo < 0) lo = b; hi = b; }
+    count_num[c] = DSMC_VIRTUAL_RECORDS;
+    valid[c] = 0;
+    if (lo < 0 || N <= (unsigned long long)(min_samples > 0 ? min_samples : 0)) return;
+    const int npts = hi - lo + 1;
+    if (npts < degree + 2) return;
+    double x[64], y[64], cf[7];
+    unsigned long long acc = 0;
+    for (int k = hi; k >= lo; k--) {
+        acc += h[k];
+        x[k - lo] = ((double)(k - lo) + 0.5) / (double)npts;
+        y[k - lo] = (double)DSMC_VIRTUAL_RECORDS * (double)acc / (double)N;
+    }
+    if (polyfit_dev(x, y, npts, degree, cf) != 0) return;
+    for (int i = 0; i <= degree; i++) coeffs[(size_t)c * (degree + 1) + i] = cf[i];
+    pmin[c] = (double)lo * dv;
+    pmax[c] = (double)(hi + 1) * dv;
+    valid[c] = 1;
+}
+
+/* n sigma per cell where a partner fit exists (collisions need a partner). */
+__global__ void nsig_kernel(const double *ndens, const int32_t *valid, double sigma, int ncell, double *nsig)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    nsig[c] = valid[c] ? ndens[c] * sigma : 0.0;
+}
+
 using namespace dsmc;
+
+/* Physical phase-2 tables: fits from the histograms (vhist_dev: device array,
+ * or NULL to upload in->vhist), n sigma from in->ndens. */
+static int stage_phase2_phys(const dsmc_params *p, const dsmc_phase2_input *in, const unsigned long long *vhist_dev,
+                             RunDev *rd, hipStream_t st)
+{
+    const int ncell = p->ftd * p->ffd * p->hd, deg = p->fit_degree, bins = p->vhist_bins;
+    int rc;
+    DevBuf vh, nd;
+    if (!vhist_dev) {
+        if ((rc = vh.upload(in->vhist, sizeof(int64_t) * (size_t)ncell * bins, st))) return rc;
+        vhist_dev = (const unsigned long long *)vh.ptr;
+    }
+    if ((rc = nd.upload(in->ndens, sizeof(double) * ncell, st))) return rc;
+    if ((rc = rd->count_num.zeros(sizeof(int32_t) * ncell, st))) return rc;
+    if ((rc = rd->coeffs.zeros(sizeof(double) * (size_t)ncell * (deg + 1), st))) return rc;
+    if ((rc = rd->pmin.zeros(sizeof(double) * ncell, st))) return rc;
+    if ((rc = rd->pmax.zeros(sizeof(double) * ncell, st))) return rc;
+    if ((rc = rd->pvalid.zeros(sizeof(int32_t) * ncell, st))) return rc;
+    if ((rc = rd->nsig.alloc(sizeof(double) * ncell))) return rc;
+    const double dv = p->vhist_max / p->vhist_bins;
+    hipLaunchKernelGGL(hist_fit_kernel, dim3((ncell + 63) / 64), dim3(64), 0, st, vhist_dev, ncell, bins, dv, deg,
+                       p->min_fit_samples, (double *)rd->coeffs.ptr, (double *)rd->pmin.ptr, (double *)rd->pmax.ptr,
+                       (int32_t *)rd->pvalid.ptr, (int32_t *)rd->count_num.ptr);
+    HIP_OK(hipGetLastError());
+    hipLaunchKernelGGL(nsig_kernel, dim3((ncell + 255) / 256), dim3(256), 0, st, (const double *)nd.ptr,
+                       (const int32_t *)rd->pvalid.ptr, p->cross_section, ncell, (double *)rd->nsig.ptr);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(st));   /* temporaries are freed on return */
+    DsmcEnv &E = rd->E;
+    E.count_num = (const int32_t *)rd->count_num.ptr;
+    E.coeffs = (const double *)rd->coeffs.ptr;
+    E.pmin = (const double *)rd->pmin.ptr;
+    E.pmax = (const double *)rd->pmax.ptr;
+    E.pvalid = (const int32_t *)rd->pvalid.ptr;
+    E.nsig = (const double *)rd->nsig.ptr;
+    E.fit_deg = deg;
+    E.copy_atm = nullptr;
+    E.atm_total = 0;
+    return DSMC_OK;
+}
+
+static int check_phys_coll(const dsmc_params *p)
+{
+    if (!p || p->mode != DSMC_MODE_PHYSICAL || p->fit_degree < 1 || p->fit_degree > 6 || p->vhist_bins <= 0 ||
+        p->vhist_bins > 64 || !(p->vhist_max > 0) || !(p->cross_section >= 0))
+        return DSMC_ERR_ARG;
+    return DSMC_OK;
+}
 
 /* Stages the phase-2 tables of `in` into rd (device) and points rd->E at them. */
 static int stage_phase2(const dsmc_params *p, const dsmc_phase2_input *in, RunDev *rd, hipStream_t st)
@@ -250,6 +342,8 @@ static int stage_phase2(const dsmc_params *p, const dsmc_phase2_input *in, RunDe
 
 static int check_phase2(const dsmc_params *p, const dsmc_phase2_input *in)
 {
+    if (p && in && p->mode == DSMC_MODE_PHYSICAL)
+        return check_phys_coll(p) ? DSMC_ERR_ARG : (in->ndens && in->vhist ? DSMC_OK : DSMC_ERR_ARG);
     if (!p || !in || p->mode != DSMC_MODE_COMPAT || p->fit_degree < 1 || p->fit_degree > 6 ||
         p->max_per_cell <= 0 || p->max_per_cell > FIT_MAXN)
         return DSMC_ERR_ARG;
@@ -289,7 +383,7 @@ extern "C" int dsmc_run_advanced(const dsmc_params *p, const dsmc_surface *s, co
     HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     RunDev rd;
     rc = stage_run(p, s, r, 1, &rd, st);
-    if (!rc) rc = stage_phase2(p, in, &rd, st);
+    if (!rc) rc = p->mode == DSMC_MODE_PHYSICAL ? stage_phase2_phys(p, in, nullptr, &rd, st) : stage_phase2(p, in, &rd, st);
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
@@ -303,6 +397,127 @@ extern "C" int dsmc_run_advanced(const dsmc_params *p, const dsmc_surface *s, co
         float ms = 0.f;
         HIP_OK(hipEventElapsedTime(&ms, e0, e1));
         r->elapsed_ms = ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+extern "C" int dsmc_build_hist_fits(const dsmc_params *p, const int64_t *vhist, double *coeffs, double *min_v,
+                                    double *max_v, int32_t *valid)
+{
+    int rc = check_phys_coll(p);
+    if (rc || !vhist) return DSMC_ERR_ARG;
+    if ((rc = ensure_device())) return rc;
+    const size_t ncell = (size_t)p->ftd * p->ffd * p->hd;
+    std::vector<double> zeros(ncell, 0.0);
+    dsmc_phase2_input in;
+    memset(&in, 0, sizeof in);
+    in.vhist = vhist;
+    in.ndens = zeros.data();
+    RunDev rd;
+    memset(&rd.E, 0, sizeof rd.E);
+    hipStream_t st;
+    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    rc = stage_phase2_phys(p, &in, nullptr, &rd, st);
+    if (!rc) {
+        if (coeffs) HIP_OK(hipMemcpy(coeffs, rd.coeffs.ptr, rd.coeffs.bytes, hipMemcpyDeviceToHost));
+        if (min_v) HIP_OK(hipMemcpy(min_v, rd.pmin.ptr, 8 * ncell, hipMemcpyDeviceToHost));
+        if (max_v) HIP_OK(hipMemcpy(max_v, rd.pmax.ptr, 8 * ncell, hipMemcpyDeviceToHost));
+        if (valid) HIP_OK(hipMemcpy(valid, rd.pvalid.ptr, 4 * ncell, hipMemcpyDeviceToHost));
+    }
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+extern "C" int dsmc_number_density(const dsmc_params *p1, const dsmc_surface *s, const int64_t *vcount,
+                                   double *ndens)
+{
+    if (!p1 || !s || !vcount || !ndens || p1->mode != DSMC_MODE_PHYSICAL || !s->face_w || s->levels < 0 ||
+        s->levels > 10 || p1->nparticles <= 0 || p1->ftd <= 0 || p1->ffd <= 0 || p1->hd <= 0)
+        return DSMC_ERR_ARG;
+    const int nfine = 20 << (2 * s->levels);
+    double F = 0.0;
+    for (int f = 0; f < nfine; f++) F += s->face_w[f];
+    const double R = p1->radius;
+    F = F * R * R * p1->sublimation_scale;            /* molecules per second */
+    const int ftd = p1->ftd, ffd = p1->ffd, hd = p1->hd;
+    std::vector<double> redge(hd + 1), sl(ftd + 1);
+    for (int k = 0; k <= hd; k++)
+        redge[k] = p1->exp_grid ? R + p1->unih * (pow(1.0 + p1->alpha2, k) - 1.0) / p1->alpha2 : R + p1->unih * k;
+    for (int t = 0; t <= ftd; t++) sl[t] = sin(-M_PI / 2.0 + M_PI * t / ftd);
+    const double dphi = 2.0 * M_PI / ffd, N = (double)p1->nparticles;
+    for (int ht = 0; ht < hd; ht++) {
+        const double r0 = redge[ht], r1 = redge[ht + 1];
+        const double shell = (r1 * r1 * r1 - r0 * r0 * r0) / 3.0;
+        for (int td = 0; td < ftd; td++) {
+            const double V = shell * (sl[td + 1] - sl[td]) * dphi;
+            for (int fd = 0; fd < ffd; fd++) {
+                const size_t c = ((size_t)ht * ftd + td) * ffd + fd;
+                ndens[c] = (double)vcount[c] * p1->dt * F / (N * V);
+            }
+        }
+    }
+    return DSMC_OK;
+}
+
+extern "C" int dsmc_run_collisional(const dsmc_params *p1, const dsmc_params *p2, const dsmc_surface *s,
+                                    dsmc_result *r1, dsmc_result *r2)
+{
+    int rc = check_phys_coll(p1);
+    if (rc || check_phys_coll(p2) || !s || !r2 || p1->ftd != p2->ftd || p1->ffd != p2->ffd || p1->hd != p2->hd ||
+        p1->vhist_bins != p2->vhist_bins || p1->vhist_max != p2->vhist_max || p1->nparticles <= 0 ||
+        p2->nparticles < 0)
+        return DSMC_ERR_ARG;
+    if ((rc = ensure_device())) return rc;
+    const size_t ncell = (size_t)p1->ftd * p1->ffd * p1->hd;
+    hipStream_t st;
+    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    /* phase 1: the caller's outputs plus counts and histograms */
+    dsmc_result spec1;
+    if (r1) spec1 = *r1;
+    else memset(&spec1, 0, sizeof spec1);
+    std::vector<int64_t> cnt(ncell);
+    spec1.vcount = cnt.data();
+    int64_t dummy = 0;
+    if (!spec1.vhist) spec1.vhist = &dummy;            /* non-NULL: keep histograms */
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    {
+        RunDev rd1;
+        rc = stage_run(p1, s, &spec1, 0, &rd1, st);
+        if (!rc) HIP_OK(hipEventRecord(e0, st));
+        if (!rc) rc = launch_run(&rd1, nullptr, st);
+        if (!rc) {
+            dsmc_result out = spec1;
+            if (out.vhist == &dummy) out.vhist = nullptr;
+            rc = collect_run(&rd1, p1, &out, st);
+            if (!rc && r1) {
+                int64_t *keep_cnt = r1->vcount;
+                *r1 = out;
+                r1->vcount = keep_cnt;
+                if (keep_cnt) memcpy(keep_cnt, cnt.data(), ncell * sizeof(int64_t));
+            }
+        }
+        std::vector<double> nd(ncell);
+        if (!rc) rc = dsmc_number_density(p1, s, cnt.data(), nd.data());
+        /* phase 2, with the phase-1 histograms still on the device */
+        RunDev rd2;
+        if (!rc) rc = stage_run(p2, s, r2, 1, &rd2, st);
+        dsmc_phase2_input in;
+        memset(&in, 0, sizeof in);
+        in.ndens = nd.data();
+        if (!rc) rc = stage_phase2_phys(p2, &in, (const unsigned long long *)rd1.vhist.ptr, &rd2, st);
+        if (!rc) rc = launch_run(&rd2, nullptr, st);
+        if (!rc) HIP_OK(hipEventRecord(e1, st));
+        if (!rc) rc = collect_run(&rd2, p2, r2, st);
+    }
+    if (!rc) {
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        r2->elapsed_ms = ms;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

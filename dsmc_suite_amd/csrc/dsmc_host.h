@@ -32,7 +32,7 @@ struct RunDev {
     DevBuf density, horizon, vmom, vcount, ttime, end_state, steps, fate, hops;
     DevBuf vrec, vrec_i, count_nn, ctl;
     /* advanced */
-    DevBuf atm, count_num, coeffs, pmin, pmax, pvalid, speeds;
+    DevBuf atm, count_num, coeffs, pmin, pmax, pvalid, speeds, nsig, vhist;
     /* particle records and slot queues */
     DevBuf rec, flyq0, flyq1, svcq, qctl, counters, rec_total, env_d;
     unsigned int *host_ctl = nullptr;     /* pinned readback of qctl */

@@ -43,8 +43,10 @@
 
 /* FLIGHT: in a hop; LANDED: hop ended on the surface (or the per-hop limit);
  * ENDED: the particle's fate was decided during a step and its outputs are
- * still to be written by the service; DONE: finished. */
-enum { LANE_IDLE = 0, LANE_FLIGHT = 1, LANE_LANDED = 2, LANE_DONE = 3, LANE_ENDED = 4 };
+ * still to be written by the service; DONE: finished; COLLIDE: a physical
+ * collisional particle reached its collision depth (service scatters it). */
+enum { LANE_IDLE = 0, LANE_FLIGHT = 1, LANE_LANDED = 2, LANE_DONE = 3, LANE_ENDED = 4, LANE_COLLIDE = 5 };
+enum { DSMC_VIRTUAL_RECORDS = 50 };   /* physical fits: P(vmin) = 50, as the reference's 50 records */
 enum { DSMC_VT_NT = 23, DSMC_VT_NV = 1000 };
 #define DSMC_FD_MARGIN 1e-12
 
@@ -82,6 +84,12 @@ struct DsmcEnv {
     const double *coeffs;     /* [cells*(deg+1)] */
     const double *pmin, *pmax;
     const int32_t *pvalid;
+    /* physical collisional: n*sigma per cell [1/m], 0 where no partner fit */
+    const double *nsig;
+    /* physical phase 1: steps per speed bin of every cell visit, [cell][vhist_bins] */
+    unsigned long long *vhist;
+    int vhist_bins;
+    double vhist_inv;         /* vhist_bins / vhist_max */
     /* outputs */
     int density_from_count;   /* physical mode with counts: W == 1, density = vcount */
     double *density, *horizon, *vmom;
@@ -112,6 +120,7 @@ struct DsmcStepEnv {
     unsigned long long *vcount;
     int hr_off;               /* this run's radial thresholds within the pool's table */
     int pad_;
+    const double *nsig;       /* collisional runs: n*sigma per cell */
 };
 
 struct DsmcLane {
@@ -125,9 +134,16 @@ struct DsmcLane {
     uint32_t rb0, rb1, rb2, rb3;
     int phase, fate, hops, num_col, td, fd, ht, acc_cell, acc_n, rec_key, rec_full, rotated;
     int face;                        /* physical: face of the last launch */
+    double ncur;                     /* physical collisional: n*sigma of cell ncell */
+    int ncell;
 };
 
-/* A particle between kernels: one 256-byte record per slot in device memory. */
+/* A particle between kernels: one 256-byte record per slot in device memory.
+ * Physical records use the compat-only slots for physical state: ii = launch
+ * face; rotated, thi, fi, lsx, lsy, lsz = the open deposit visit (cell, weight
+ * sum, step count, velocity sums), kept across launches so that a launch
+ * boundary never splits a visit; real_col_p, col_p = collision depth and its
+ * threshold (collisional runs). */
 struct DsmcRec {
     double sx, sy, sz, vx, vy, vz, ox, oy, oz;
     double Ri, W, time, flight_time, thi, fi, rest_time, real_col_p, col_p, col_lo;
@@ -142,11 +158,25 @@ struct DsmcCellTabs {
     const double *tq, *hr, *cs;
 };
 
+/* Speed-histogram bin of one cell visit (n deposits with velocity sums vx, vy,
+ * vz): the speed of the mean velocity, vhist_bins bins up to vhist_max, the
+ * last bin open-ended.  NULL when the run keeps no histograms. */
+DSMC_HD unsigned long long *vhist_bin(const DsmcEnv &E, int cell, double vx, double vy, double vz, int n)
+{
+    if (!E.vhist || n <= 0) return nullptr;
+    double b = sqrt(vx * vx + vy * vy + vz * vz) / (double)n * E.vhist_inv;
+    int bin = b < (double)(E.vhist_bins - 1) ? (int)b : E.vhist_bins - 1;   /* NaN -> last bin */
+    return E.vhist + (size_t)cell * E.vhist_bins + bin;
+}
+DSMC_HD unsigned long long *vhist_bin(const DsmcStepEnv &, int, double, double, double, int) { return nullptr; }
+
 /* Atomic/plain accumulation, supplied by the caller (GPU atomics or host adds). */
 struct DsmcOpsHost {
     template <class EnvT>
     static inline void deposit(const EnvT &E, int cell, double w, double vx, double vy, double vz, int n)
     {
+        unsigned long long *hb = vhist_bin(E, cell, vx, vy, vz, n);
+        if (hb) *hb += (unsigned long long)n;
         if (!E.density_from_count) E.density[cell] += w;
         if (E.vmom) {
             E.vmom[(size_t)cell * 3 + 0] += vx;
@@ -371,6 +401,20 @@ DSMC_HD void counts_flush(DsmcCounts &cnt, const DsmcEnv &E)
 }
 
 /* ------------------------------------------------------------------ records */
+/* The open deposit visit of a physical particle, kept in its record. */
+DSMC_HD void rec_load_visit(DsmcLane &L, const DsmcRec &R)
+{
+    L.acc_cell = R.rotated;
+    L.acc_w = R.thi; L.acc_n = (int)R.fi;
+    L.acx = R.lsx; L.acy = R.lsy; L.acz = R.lsz;
+}
+DSMC_HD void rec_store_visit(const DsmcLane &L, DsmcRec &R)
+{
+    R.rotated = L.acc_cell;
+    R.thi = L.acc_w; R.fi = (double)L.acc_n;
+    R.lsx = L.acx; R.lsy = L.acy; R.lsz = L.acz;
+}
+
 /* Only what a flight step reads or writes travels with the flight kernel. */
 template <int MODE, int ADV>
 DSMC_HD void rec_load_flight(DsmcLane &L, const DsmcRec &R)
@@ -387,7 +431,8 @@ DSMC_HD void rec_load_flight(DsmcLane &L, const DsmcRec &R)
         L.real_col_p = R.real_col_p; L.col_p = R.col_p; L.col_lo = R.col_lo;
         L.num_col = R.num_col; L.k = R.k; L.pid = R.pid; L.blk = ~0ull;
     }
-    L.acc_cell = -1; L.rec_key = -1; L.rec_full = 0;
+    L.acc_cell = -1; L.rec_key = -1; L.rec_full = 0; L.ncell = -1;
+    if (MODE != 0) rec_load_visit(L, R);
 }
 
 template <int MODE, int ADV>
@@ -400,8 +445,8 @@ DSMC_HD void rec_store_flight(const DsmcLane &L, DsmcRec &R)
     R.td = L.td; R.fd = L.fd; R.ht = L.ht;
     R.phase = L.phase; R.fate = L.fate;
     if (MODE == 0) { R.time = L.time; R.flight_time = L.flight_time; R.ii = (int32_t)L.ii; }
-    else { R.n_day = L.n_day; }
-    if (ADV) { R.real_col_p = L.real_col_p; R.num_col = L.num_col; R.k = L.k; }
+    else { R.n_day = L.n_day; rec_store_visit(L, R); }
+    if (ADV) { R.real_col_p = L.real_col_p; R.col_p = L.col_p; R.num_col = L.num_col; R.k = L.k; }
 }
 
 /* The whole particle (start and service kernels).  The record's ii slot holds
@@ -419,7 +464,8 @@ DSMC_HD void rec_load_all(DsmcLane &L, const DsmcRec &R)
     else { L.ii = 0; L.face = R.ii; }
     L.phase = R.phase; L.fate = R.fate; L.hops = R.hops; L.num_col = R.num_col;
     L.td = R.td; L.fd = R.fd; L.ht = R.ht; L.rotated = R.rotated;
-    L.blk = ~0ull; L.acc_cell = -1; L.rec_key = -1; L.rec_full = 0;
+    L.blk = ~0ull; L.acc_cell = -1; L.rec_key = -1; L.rec_full = 0; L.ncell = -1;
+    if (MODE != 0) { L.rotated = 0; rec_load_visit(L, R); }
 }
 
 template <int MODE>
@@ -434,6 +480,7 @@ DSMC_HD void rec_store_all(const DsmcLane &L, DsmcRec &R, int32_t run)
     R.ii = MODE == 0 ? (int32_t)L.ii : L.face;
     R.phase = L.phase; R.fate = L.fate; R.hops = L.hops; R.num_col = L.num_col;
     R.td = L.td; R.fd = L.fd; R.ht = L.ht; R.rotated = L.rotated; R.run = run;
+    if (MODE != 0) rec_store_visit(L, R);
 }
 
 /* What a physical particle carries between launches of the flight kernel with
@@ -447,8 +494,10 @@ DSMC_HD void rec_load_phys(DsmcLane &L, const DsmcRec &R)
     L.phase = R.phase; L.fate = R.fate; L.hops = R.hops;
     L.td = R.td; L.fd = R.fd; L.ht = R.ht;
     L.face = R.ii;                   /* physical records keep the launch face in ii */
-    L.num_col = 0; L.rotated = 0;
-    L.blk = ~0ull; L.acc_cell = -1; L.rec_key = -1; L.rec_full = 0;
+    L.num_col = R.num_col; L.rotated = 0;
+    L.real_col_p = R.real_col_p; L.col_p = R.col_p;
+    L.blk = ~0ull; L.rec_key = -1; L.rec_full = 0; L.ncell = -1;
+    rec_load_visit(L, R);
 }
 
 DSMC_HD void rec_store_phys(const DsmcLane &L, DsmcRec &R)
@@ -460,6 +509,8 @@ DSMC_HD void rec_store_phys(const DsmcLane &L, DsmcRec &R)
     R.phase = L.phase; R.fate = L.fate; R.hops = L.hops;
     R.td = L.td; R.fd = L.fd; R.ht = L.ht;
     R.ii = L.face;
+    R.num_col = L.num_col; R.real_col_p = L.real_col_p; R.col_p = L.col_p;
+    rec_store_visit(L, R);
 }
 
 /* ------------------------------------------------------------------ hop start */
@@ -656,6 +707,63 @@ DSMC_HD int locate_face_near(const DsmcEnv &E, int hint, double dx, double dy, d
     return locate_face(E, dx, dy, dz);
 }
 
+DSMC_HD double poly_eval(const double *c, int deg, double x)
+{
+    double v = 0.0;
+    for (int i = 0; i <= deg; i++) v = v * x + c[i];
+    return v;
+}
+
+/* Physical partner speed: the fit P(x) of the reverse cumulative speed
+ * histogram in normalised speed x = (v - min_v) / (max_v - min_v), P(0) = 50;
+ * the speed with P(x) = u (50 - P(1)) + P(1), by bisection (the sign at the
+ * left end tracked, an exact zero ends it) to a 0.01 m/s bracket. */
+DSMC_HD double sample_speed_phys(const double *c, int deg, double min_v, double max_v, double u)
+{
+    const double pv = poly_eval(c, deg, 1.0);
+    const double span = max_v - min_v;
+    const double ranking = u * ((double)DSMC_VIRTUAL_RECORDS - pv) + pv;
+    double x0 = 0.0, x1 = 1.0;
+    double f0 = poly_eval(c, deg, 0.0) - ranking;
+    for (int iter = 0; iter < 100; iter++) {
+        double xm = (x0 + x1) * 0.5;
+        double fm = poly_eval(c, deg, xm) - ranking;
+        if (fm == 0.0) { x0 = x1 = xm; break; }
+        if ((fm < 0.0) == (f0 < 0.0)) { x0 = xm; f0 = fm; }
+        else x1 = xm;
+        if ((x1 - x0) * span < 0.01) break;
+    }
+    return min_v + (x0 + x1) * 0.5 * span;
+}
+
+/* Elastic hard-sphere collision with a partner of the cell's speed distribution
+ * moving in an isotropic direction; equal masses, isotropic scattering in the
+ * centre-of-mass frame.  Then a new collision depth threshold. */
+DSMC_HD void lane_collide(DsmcLane &L, const DsmcEnv &E)
+{
+    if (L.ht >= E.hd) { L.real_col_p = 0.0; L.phase = LANE_FLIGHT; return; }   /* cannot happen: depth grows in-grid */
+    const int c = (L.ht * E.ftd + L.td) * E.ffd + L.fd;
+    const int deg = E.fit_deg;
+    double u = lane_u01(L, E);
+    double vv2 = sample_speed_phys(E.coeffs + (size_t)c * (deg + 1), deg, E.pmin[c], E.pmax[c], u);
+    double ct = 2.0 * lane_u01(L, E) - 1.0, st = sqrt(1.0 - ct * ct);
+    double sp, cp;
+    dm_sincos(2.0 * M_PI * lane_u01(L, E), &sp, &cp);
+    double v2x = vv2 * st * cp, v2y = vv2 * st * sp, v2z = vv2 * ct;
+    double gx = L.vx - v2x, gy = L.vy - v2y, gz = L.vz - v2z;
+    double g = sqrt(gx * gx + gy * gy + gz * gz);
+    double ct2 = 2.0 * lane_u01(L, E) - 1.0, st2 = sqrt(1.0 - ct2 * ct2);
+    double sp2, cp2;
+    dm_sincos(2.0 * M_PI * lane_u01(L, E), &sp2, &cp2);
+    L.vx = 0.5 * (L.vx + v2x) + 0.5 * g * st2 * cp2;
+    L.vy = 0.5 * (L.vy + v2y) + 0.5 * g * st2 * sp2;
+    L.vz = 0.5 * (L.vz + v2z) + 0.5 * g * ct2;
+    L.num_col++;
+    L.real_col_p = 0.0;
+    L.col_p = -dm_log(lane_u01(L, E));
+    L.phase = LANE_FLIGHT;
+}
+
 /* ------------------------------------------------------------------ particle start */
 template <class Ops, int MODE, int ADV>
 DSMC_HD void lane_start(DsmcLane &L, const DsmcEnv &E, int64_t pid, DsmcCounts &cnt)
@@ -720,6 +828,11 @@ DSMC_HD void lane_start(DsmcLane &L, const DsmcEnv &E, int64_t pid, DsmcCounts &
     L.n_photo = nph > 9.0e18 ? (int64_t)9000000000000000000LL : (int64_t)nph;
     L.face = face;
     lane_emit(L, E, E.face_T[face]);
+    if (ADV) {                       /* collision depth threshold ~ Exp(1) */
+        L.real_col_p = 0.0;
+        L.col_p = -dm_log(lane_u01(L, E));
+    }
+    L.ncell = -1;
     L.hops = 1;
     lane_begin_flight(L, E);
 }
@@ -743,6 +856,7 @@ DSMC_HD void lane_service(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
         lane_launch_compat<Ops, ADV>(L, E, cnt);
         return;
     }
+    if (ADV && L.phase == LANE_COLLIDE) { lane_collide(L, E); return; }
     int64_t cap = E.max_steps > 0 ? E.max_steps : INT64_MAX;
     if (L.steps >= cap) { L.fate = 5; lane_finish<MODE>(L, E, cnt); return; }
     const double Rl = sqrt(L.sx * L.sx + L.sy * L.sy + L.sz * L.sz);
@@ -767,12 +881,6 @@ DSMC_HD void lane_service(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
 }
 
 /* ------------------------------------------------------------------ collision (advanced) */
-DSMC_HD double poly_eval(const double *c, int deg, double x)
-{
-    double v = 0.0;
-    for (int i = 0; i <= deg; i++) v = v * x + c[i];
-    return v;
-}
 
 /* collision.c:43-101 */
 DSMC_HD double sample_speed(const double *c, int deg, double min_v, double max_v, double u, int nn)
@@ -834,9 +942,20 @@ DSMC_HD void lane_step(DsmcLane &L, const EnvT &E, const DsmcCellTabs &TB)
         L.td = td_update(nz, Rn, L.td, E.ftd, TB.tq);
         L.fd = fd_update(nx, ny, L.fd, E.ffd, TB.cs);
         L.ht = table_walk(TB.hr, L.ht, E.hd, Rn);
-        if (L.ht < E.hd) lane_deposit<Ops>(L, E, (L.ht * E.ftd + L.td) * E.ffd + L.fd);
+        if (L.ht < E.hd) {
+            const int cell = (L.ht * E.ftd + L.td) * E.ffd + L.fd;
+            if constexpr (ADV != 0) {
+                if (cell != L.ncell) { L.ncur = E.nsig[cell]; L.ncell = cell; }
+            }
+            lane_deposit<Ops>(L, E, cell);
+        } else if constexpr (ADV != 0) {
+            L.ncur = 0.0;
+            L.ncell = -1;
+        }
+        if constexpr (ADV != 0) L.real_col_p += L.ncur * ds;     /* collision depth */
         int64_t cap = E.max_steps > 0 ? E.max_steps : INT64_MAX;
         if (L.steps >= cap) { lane_flush<Ops>(L, E); L.fate = 5; L.phase = LANE_ENDED; }
+        else if (ADV != 0 && L.real_col_p >= L.col_p) L.phase = LANE_COLLIDE;
     } else {
         lane_step_compat<Ops, ADV>(L, E, TB, nx, ny, nz, Rn, ds);
     }

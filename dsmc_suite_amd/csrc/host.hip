@@ -137,6 +137,13 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
         E.vcount = (unsigned long long *)rd->vcount.ptr;
         E.density_from_count = p->mode == DSMC_MODE_PHYSICAL;
     }
+    if (r->vhist && p->mode == DSMC_MODE_PHYSICAL && !advanced) {
+        if (p->vhist_bins <= 0 || p->vhist_bins > 64 || !(p->vhist_max > 0)) return DSMC_ERR_ARG;
+        if ((rc = rd->vhist.zeros(cells * p->vhist_bins * sizeof(unsigned long long), st))) return rc;
+        E.vhist = (unsigned long long *)rd->vhist.ptr;
+        E.vhist_bins = p->vhist_bins;
+        E.vhist_inv = (double)p->vhist_bins / p->vhist_max;
+    }
     int rows = advanced ? 9 : 5;
     E.ttime_rows = rows;
     if (r->ttime && p->mode == DSMC_MODE_COMPAT) {
@@ -228,7 +235,8 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
     /* per-launch step budget: long hops are split over rounds so the run's tail is
      * set by its longest particle, not by the sum of each round's longest hop */
     const int64_t sh_budget = sh.step_budget;
-    const bool inline_svc = sh.service_inline && rd->E.mode == DSMC_MODE_PHYSICAL && !rd->E.advanced;
+    /* physical collisional runs always service inline (collisions are service events) */
+    const bool inline_svc = rd->E.mode == DSMC_MODE_PHYSICAL && (sh.service_inline || rd->E.advanced);
     std::vector<hipEvent_t> ev;
     int p = 0, chunk = 4;
     for (;;) {
@@ -317,6 +325,8 @@ int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st
     if (r->steps) HIP_OK(hipMemcpyAsync(r->steps, rd->steps.ptr, 8 * n, hipMemcpyDeviceToHost, st));
     if (r->fate) HIP_OK(hipMemcpyAsync(r->fate, rd->fate.ptr, 4 * n, hipMemcpyDeviceToHost, st));
     if (r->hops) HIP_OK(hipMemcpyAsync(r->hops, rd->hops.ptr, 4 * n, hipMemcpyDeviceToHost, st));
+    if (r->vhist && rd->vhist.ptr)
+        HIP_OK(hipMemcpyAsync(r->vhist, rd->vhist.ptr, rd->vhist.bytes, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     if (rd->E.density_from_count && r->density) {
         const int64_t *c = r->vcount ? r->vcount : (const int64_t *)cnt_tmp.data();
@@ -429,7 +439,12 @@ extern "C" int dsmc_params_preset(dsmc_params *p, int preset, int ftd, int ffd)
         p->record_cap_total = 0;
         p->cross_section = M_PI * 4.6e-10 * 4.6e-10;
         p->fit_degree = 6;
-        if (preset == DSMC_PRESET_CERES_PHYSICAL_COLL) p->sublimation_scale = 10.0;
+        p->vhist_bins = 64;                  /* phase-1 speed histograms: 31.25 m/s bins */
+        p->vhist_max = 2000.0;
+        if (preset == DSMC_PRESET_CERES_PHYSICAL_COLL) {
+            p->sublimation_scale = 10.0;
+            p->stream = DSMC_STREAM_ADVANCED;   /* phase 2 draws apart from phase 1 */
+        }
         return DSMC_OK;
     }
     return DSMC_ERR_ARG;
@@ -497,7 +512,7 @@ extern "C" dsmc_run *dsmc_run_create(const dsmc_params *p, const dsmc_surface *s
 static int clear_outputs(RunDev &rd, hipStream_t st)
 {
     DevBuf *bufs[] = {&rd.density, &rd.horizon, &rd.vmom, &rd.vcount, &rd.ttime, &rd.end_state,
-                      &rd.steps, &rd.fate, &rd.hops, &rd.count_nn, &rd.counters, &rd.rec_total};
+                      &rd.steps, &rd.fate, &rd.hops, &rd.count_nn, &rd.counters, &rd.rec_total, &rd.vhist};
     for (DevBuf *b : bufs)
         if (b->ptr) HIP_OK(hipMemsetAsync(b->ptr, 0, b->bytes, st));
     return DSMC_OK;

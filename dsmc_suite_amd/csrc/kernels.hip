@@ -47,11 +47,20 @@ __device__ static inline void grid_deposit(const EnvT &E, int cell, double w, do
     if (E.vcount) atomicAdd(&E.vcount[cell], (unsigned long long)n);
 }
 
+/* The speed histogram of a visit (physical runs that keep one). */
+template <class EnvT>
+__device__ static inline void visit_hist(const EnvT &E, int cell, double vx, double vy, double vz, int n)
+{
+    unsigned long long *hb = vhist_bin(E, cell, vx, vy, vz, n);
+    if (hb) atomicAdd(hb, (unsigned long long)n);
+}
+
 struct DsmcOpsDev {
     template <class EnvT>
     __device__ static inline void deposit(const EnvT &E, int cell, double w, double vx, double vy,
                                           double vz, int n)
     {
+        visit_hist(E, cell, vx, vy, vz, n);
         grid_deposit(E, cell, w, vx, vy, vz, n);
     }
     __device__ static inline void add_f64(double *p, double v) { atomicAdd(p, v); }
@@ -77,6 +86,7 @@ struct DsmcOpsLds {
     __device__ static inline void deposit(const EnvT &E, int cell, double w, double vx, double vy,
                                           double vz, int n)
     {
+        visit_hist(E, cell, vx, vy, vz, n);
         int s = dep_slot(cell);
         int t = dep_tag[s];
         if (t == -1) {
@@ -240,7 +250,8 @@ flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch, in
         bool hop_end = L.phase == LANE_LANDED || L.phase == LANE_ENDED;
         bool paused = L.phase == LANE_IDLE + 8;
         if (hop_end || paused) {
-            lane_flush<DsmcOpsLds>(L, E);
+            /* physical records keep an open visit across launches */
+            if (hop_end || MODE == 0) lane_flush<DsmcOpsLds>(L, E);
             if (paused) L.phase = LANE_FLIGHT;
             rec_store_flight<MODE, ADV>(L, rec[slot]);
         }
@@ -319,7 +330,7 @@ pool_flight_kernel(DsmcPoolDev P, DsmcQueues Q, int p, int refill_batch, int64_t
         bool hop_end = L.phase == LANE_LANDED || L.phase == LANE_ENDED;
         bool paused = L.phase == LANE_IDLE + 8;
         if (hop_end || paused) {
-            lane_flush<DsmcOpsDev>(L, S);
+            if (hop_end) lane_flush<DsmcOpsDev>(L, S);     /* pauses keep the open visit */
             if (paused) L.phase = LANE_FLIGHT;
             rec_store_flight<MODE, ADV>(L, rec[slot]);
         }
@@ -333,11 +344,11 @@ pool_flight_kernel(DsmcPoolDev P, DsmcQueues Q, int p, int refill_batch, int64_t
  * those of the Verlet step; the lane state travels through the call frame.  The
  * run's environment is copied from device memory once, at the top (its fields
  * load together instead of one dependent load per use). */
-template <class Ops>
+template <class Ops, int ADV>
 __device__ __attribute__((noinline)) void phys_service(DsmcLane &T, const DsmcEnv *Ep, DsmcCounts &cnt)
 {
     const DsmcEnv E = *Ep;
-    lane_service<Ops, 1, 0>(T, E, cnt);
+    lane_service<Ops, 1, ADV>(T, E, cnt);
 }
 
 /* Physical particles: flight with the landing service inline.  A lane whose hop
@@ -349,7 +360,7 @@ __device__ __attribute__((noinline)) void phys_service(DsmcLane &T, const DsmcEn
  * particles of different runs (step constants in registers, every run's radial
  * thresholds in LDS); otherwise one run's environment is the kernel argument and
  * deposits go through the LDS slot table. */
-template <bool POOL>
+template <bool POOL, int ADV>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 phys_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcPoolDev P, DsmcRec *rec, DsmcQueues Q, int p, int svc_batch,
             int64_t step_budget)
@@ -410,31 +421,30 @@ phys_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcPoolDev P, DsmcRec *rec, DsmcQueu
         if (POOL) { S.ftd = P.ftd; S.ffd = P.ffd; S.hd = P.hd; }
         for (;;) {
             if (L.phase == LANE_FLIGHT) {
-                if (POOL) lane_step<DsmcOpsDev, 1, 0>(L, S, TB);
-                else lane_step<DsmcOpsLds, 1, 0>(L, E0, TB);
+                if (POOL) lane_step<DsmcOpsDev, 1, ADV>(L, S, TB);
+                else lane_step<DsmcOpsLds, 1, ADV>(L, E0, TB);
                 if (L.phase == LANE_FLIGHT && L.steps >= budget_end) L.phase = LANE_IDLE + 8;
             }
             uint64_t ended = __ballot(L.phase != LANE_FLIGHT && L.phase != LANE_IDLE);
             uint64_t waiting = ended | __ballot(L.phase == LANE_IDLE);
             if ((ended && __popcll(waiting) >= svc_batch) || waiting == __ballot(true)) break;
         }
-        /* landings and ended particles: service inline */
+        /* landings, collisions and ended particles: service inline */
         DsmcCounts cnt = {0, 0, 0, 0, 0, 0, 0, 0};
-        const bool svc = L.phase == LANE_LANDED || L.phase == LANE_ENDED;
+        const bool svc = L.phase == LANE_LANDED || L.phase == LANE_ENDED || L.phase == LANE_COLLIDE;
         if (svc) {
             DsmcLane T = L;
-            if (POOL) phys_service<DsmcOpsDev>(T, P.envs + run, cnt);
-            else phys_service<DsmcOpsLds>(T, Ed, cnt);
+            if (POOL) phys_service<DsmcOpsDev, ADV>(T, P.envs + run, cnt);
+            else phys_service<DsmcOpsLds, ADV>(T, Ed, cnt);
             L = T;
         }
         const bool fin = svc && L.phase == LANE_DONE;
         if (__ballot(fin)) counts_flush_wave(cnt, fin ? (POOL ? P.envs[run].counters : E0.counters) : nullptr, fin);
         if (fin) L.phase = LANE_IDLE;
-        /* budget-paused particles go to the next launch */
+        /* budget-paused particles go to the next launch, their open deposit
+         * visit with them (a launch boundary does not split a visit) */
         bool paused = L.phase == LANE_IDLE + 8;
         if (paused) {
-            if (POOL) lane_flush<DsmcOpsDev>(L, S);
-            else lane_flush<DsmcOpsLds>(L, E0);
             L.phase = LANE_FLIGHT;
             rec_store_phys(L, rec[slot]);
         }
@@ -552,7 +562,7 @@ __global__ void dsmc_math_kernel(int fn, const double *x, const double *y, doubl
         if (mode == 0 && !adv) hipLaunchKernelGGL((KERNEL<0, 0>), __VA_ARGS__);               \
         else if (mode == 0) hipLaunchKernelGGL((KERNEL<0, 1>), __VA_ARGS__);                  \
         else if (!adv) hipLaunchKernelGGL((KERNEL<1, 0>), __VA_ARGS__);                       \
-        else return hipErrorInvalidValue;                                                     \
+        else hipLaunchKernelGGL((KERNEL<1, 1>), __VA_ARGS__);                                 \
     } while (0)
 
 static bool block_ok(int block) { return block >= 64 && block <= 256 && block % 64 == 0; }
@@ -610,15 +620,19 @@ extern "C" hipError_t dsmc_launch_phys(const DsmcEnv *E, const DsmcEnv *Ed, cons
         const size_t lds = sizeof(double) * (size_t)((P->ftd + 1) + 2 * (P->ffd + 1) + P->nhr);
         DsmcEnv none;
         memset(&none, 0, sizeof none);
-        hipLaunchKernelGGL((phys_kernel<true>), dim3(grid), dim3(block), lds, st, none, (const DsmcEnv *)nullptr, *P,
+        hipLaunchKernelGGL((phys_kernel<true, 0>), dim3(grid), dim3(block), lds, st, none, (const DsmcEnv *)nullptr, *P,
                            rec, Q, p, svc_batch, step_budget);
     } else {
-        if (E->mode != 1 || E->advanced || !Ed) return hipErrorInvalidValue;
+        if (E->mode != 1 || !Ed) return hipErrorInvalidValue;
         const size_t lds = sizeof(double) * (size_t)((E->ftd + 1) + 2 * (E->ffd + 1) + (E->hd + 2));
         DsmcPoolDev none;
         memset(&none, 0, sizeof none);
-        hipLaunchKernelGGL((phys_kernel<false>), dim3(grid), dim3(block), lds, st, *E, Ed, none, rec, Q, p,
-                           svc_batch, step_budget);
+        if (E->advanced)
+            hipLaunchKernelGGL((phys_kernel<false, 1>), dim3(grid), dim3(block), lds, st, *E, Ed, none, rec, Q, p,
+                               svc_batch, step_budget);
+        else
+            hipLaunchKernelGGL((phys_kernel<false, 0>), dim3(grid), dim3(block), lds, st, *E, Ed, none, rec, Q, p,
+                               svc_batch, step_budget);
     }
     hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, Q.ctl, p, Q.ctl + 8);
     return hipGetLastError();

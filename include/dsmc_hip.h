@@ -113,7 +113,12 @@ typedef struct dsmc_params {
     double density_scale;     /* multiplies the phase-1 density read as copy_atm */
 
     /* physical-mode emission */
-    double sublimation_scale; /* multiplies face sublimation weights (cfg3: x10) */
+    double sublimation_scale; /* multiplies face sublimation weights (collisional runs: x10) */
+
+    /* physical collisional runs: phase-1 speed histograms per cell (one sample
+     * per cell visit: the speed of the visit's mean velocity) */
+    int vhist_bins;           /* bins per cell, <= 64 */
+    double vhist_max;         /* upper edge of the last regular bin [m/s]; the last bin is open */
 } dsmc_params;
 
 /* Presets.  COMPAT_BALLISTIC / COMPAT_ADVANCED reproduce the reference CLI
@@ -167,6 +172,7 @@ typedef struct dsmc_result {
     int64_t *steps;           /* [n] Verlet steps taken by each particle */
     int32_t *fate;            /* [n] enum dsmc_fate */
     int32_t *hops;            /* [n] launches per particle */
+    int64_t *vhist;           /* [hd*ftd*ffd*vhist_bins] physical: steps per speed bin of cell visits */
     /* compat ballistic velocity records, capacity record_cap_total */
     double *vrec;             /* [cap*3] */
     int32_t *vrec_td, *vrec_fd, *vrec_ht;
@@ -185,6 +191,10 @@ typedef struct dsmc_phase2_input {
     const double *vrec;       /* [nrec*3] or NULL (no fits) */
     const int32_t *vrec_td, *vrec_fd, *vrec_ht;
     int64_t nrec;
+    /* physical collisional runs instead: the phase-1 number density [m^-3] and
+     * speed histograms of every cell, both in the [ht][td][fd] cell order */
+    const double *ndens;
+    const int64_t *vhist;
 } dsmc_phase2_input;
 
 /* Single runs on the current device.  Inputs stay in host memory; the call
@@ -224,6 +234,26 @@ void dsmc_run_destroy(dsmc_run *run);
 int dsmc_verlet_steps(double *state, int n, int nsteps, double GM, double R, double max_height,
                       double h, double *traj);
 int dsmc_elementary(int fn, const double *x, const double *y, double *out, int n);
+
+/* Physical collisional runs.
+ *
+ * Number density of a phase-1 run: the steps spent in each cell (vcount) x dt x
+ * the body's emission rate (sum of face weights x R^2 x sublimation_scale,
+ * molecules/s) / (nparticles x cell volume); cells [ht][td][fd], spherical-shell
+ * sectors between the grid's radial, latitude and longitude edges. */
+int dsmc_number_density(const dsmc_params *p1, const dsmc_surface *s, const int64_t *vcount, double *ndens);
+/* Partner-speed fits from phase-1 histograms (device): per cell with more than
+ * min_fit_samples steps, the reverse cumulative histogram P = 50 (steps at or
+ * above the bin) / steps at the occupied bins, fitted with a degree fit_degree
+ * polynomial in the normalised speed (polyfit.c's normal equations).  coeffs
+ * [cell*(degree+1)], highest power first; min_v/max_v the occupied range. */
+int dsmc_build_hist_fits(const dsmc_params *p, const int64_t *vhist, double *coeffs, double *min_v,
+                         double *max_v, int32_t *valid);
+/* Phase 1 (p1: physical ballistic, histograms kept) then phase 2 (p2:
+ * collisional, dt and grid of p2; p2 must match p1's grid) with the phase-1
+ * density and fits handed over on the device.  r1 may be NULL. */
+int dsmc_run_collisional(const dsmc_params *p1, const dsmc_params *p2, const dsmc_surface *s, dsmc_result *r1,
+                         dsmc_result *r2);
 
 /* Per-cell polynomial CDF fits (advanced.c:32-117 + polyfit.c:10-101) computed on
  * the device from velocity records: outputs [ftd*ffd*hd] cells in the

@@ -337,6 +337,12 @@ typedef struct {
     const int *count_num;
     const double *poly_coeffs, *poly_min, *poly_max;
     const int *poly_valid;
+    /* physical collisional: phase-1 visit histograms, phase-2 n*sigma */
+    int64_t *vhist;
+    int vbins;
+    double vinv;
+    const double *nsig;
+    int coll;
     /* physical */
     const dsmc_surface *surf;
     double *cdf;
@@ -711,6 +717,90 @@ static V3 emit(const ctx_t *c, V3 s, double T, orng *rg)
     return v;
 }
 
+/* A cell visit (deposits into one cell in a row, within one hop) goes to the
+ * speed histogram as one sample: the speed of its mean velocity, weighted by
+ * its step count. */
+typedef struct { int cell; int64_t n; double sx, sy, sz; } visit_t;
+static void visit_flush(ctx_t *c, visit_t *vi)
+{
+    if (c->vhist && vi->cell >= 0 && vi->n > 0) {
+        double b = sqrt(vi->sx * vi->sx + vi->sy * vi->sy + vi->sz * vi->sz) / (double)vi->n * c->vinv;
+        int bin = b < (double)(c->vbins - 1) ? (int)b : c->vbins - 1;
+        #pragma omp atomic
+        c->vhist[(size_t)vi->cell * c->vbins + bin] += vi->n;
+    }
+    vi->cell = -1; vi->n = 0; vi->sx = vi->sy = vi->sz = 0.0;
+}
+static void visit_add(ctx_t *c, visit_t *vi, int cell, V3 v)
+{
+    if (cell != vi->cell) { visit_flush(c, vi); vi->cell = cell; }
+    vi->n++;
+    vi->sx += v.x; vi->sy += v.y; vi->sz += v.z;
+}
+
+static double poly_at(const double *cf, int deg, double x)
+{
+    double v = 0.0;
+    for (int i = 0; i <= deg; i++) v = v * x + cf[i];
+    return v;
+}
+
+/* Physical partner speed from the normalised-speed fit: P(x) = u (50 - P(1)) +
+ * P(1) by bisection on x in [0, 1] (sign of the left end tracked) until the
+ * bracket is below 0.01 m/s. */
+double cpu_ref_sample_speed_phys(const double *cf, int deg, double min_v, double max_v, double u)
+{
+    const double pv = poly_at(cf, deg, 1.0);
+    const double span = max_v - min_v;
+    const double ranking = u * (50.0 - pv) + pv;
+    double x0 = 0.0, x1 = 1.0;
+    double f0 = poly_at(cf, deg, 0.0) - ranking;
+    for (int iter = 0; iter < 100; iter++) {
+        double xm = (x0 + x1) * 0.5;
+        double fm = poly_at(cf, deg, xm) - ranking;
+        if (fm == 0.0) { x0 = x1 = xm; break; }
+        if ((fm < 0.0) == (f0 < 0.0)) { x0 = xm; f0 = fm; }
+        else x1 = xm;
+        if ((x1 - x0) * span < 0.01) break;
+    }
+    return min_v + (x0 + x1) * 0.5 * span;
+}
+
+/* Elastic hard-sphere collision of equal masses: the partner has a speed of the
+ * cell's distribution and an isotropic direction; the relative velocity turns to
+ * an isotropic direction in the centre-of-mass frame. */
+/* The kinematics alone: partner direction from u[0], u[1], outgoing relative
+ * direction from u[2], u[3] (cos theta = 2u - 1, phi = 2 pi u). */
+void cpu_ref_collide_kinematics(const double v[3], double vv2, const double u[4], double out[3], double v2[3])
+{
+    double ct = 2.0 * u[0] - 1.0, st = sqrt(1.0 - ct * ct);
+    double ph = 2.0 * M_PI * u[1];
+    double sp = OM_SIN(ph), cp = OM_COS(ph);
+    double v2x = vv2 * st * cp, v2y = vv2 * st * sp, v2z = vv2 * ct;
+    double gx = v[0] - v2x, gy = v[1] - v2y, gz = v[2] - v2z;
+    double g = sqrt(gx * gx + gy * gy + gz * gz);
+    double ct2 = 2.0 * u[2] - 1.0, st2 = sqrt(1.0 - ct2 * ct2);
+    double ph2 = 2.0 * M_PI * u[3];
+    double sp2 = OM_SIN(ph2), cp2 = OM_COS(ph2);
+    out[0] = 0.5 * (v[0] + v2x) + 0.5 * g * st2 * cp2;
+    out[1] = 0.5 * (v[1] + v2y) + 0.5 * g * st2 * sp2;
+    out[2] = 0.5 * (v[2] + v2z) + 0.5 * g * ct2;
+    if (v2) { v2[0] = v2x; v2[1] = v2y; v2[2] = v2z; }
+}
+
+static void phys_collide(const ctx_t *c, V3 *v, size_t cell, orng *rg)
+{
+    const int deg = c->fit_deg;
+    double u = orng_u01(rg);
+    double vv2 = cpu_ref_sample_speed_phys(c->poly_coeffs + cell * (deg + 1), deg, c->poly_min[cell],
+                                           c->poly_max[cell], u);
+    double uu[4];
+    for (int i = 0; i < 4; i++) uu[i] = orng_u01(rg);
+    double vin[3] = {v->x, v->y, v->z}, out[3];
+    cpu_ref_collide_kinematics(vin, vv2, uu, out, NULL);
+    v->x = out[0]; v->y = out[1]; v->z = out[2];
+}
+
 static void physical_particle(ctx_t *c, int64_t n, orng *rg)
 {
     const dsmc_params *P = c->p;
@@ -739,6 +829,10 @@ static void physical_particle(ctx_t *c, int64_t n, orng *rg)
     double nph = ceil(E * P->lifetime / h);
     int64_t n_photo = nph > 9.0e18 ? (int64_t)9000000000000000000LL : (int64_t)nph;
     V3 v = emit(c, s, c->surf->face_T[face], rg);
+    double tau = 0.0, tau_next = 0.0;          /* collision depth and its threshold */
+    int64_t ncol = 0;
+    if (c->coll) tau_next = -OM_LOG(orng_u01(rg));
+    visit_t vi = {-1, 0, 0.0, 0.0, 0.0};
     hops = 1;
     int64_t cap = P->max_steps > 0 ? P->max_steps : INT64_MAX;
 
@@ -749,6 +843,7 @@ static void physical_particle(ctx_t *c, int64_t n, orng *rg)
         int landed = 0;
         for (;;) {
             steps++;
+            V3 so = s;
             int st = verlet(&s, &v, c->GM, R, c->max_height, h, &oa, NULL);
             if (st == VS_LANDED) { landed = 1; break; }
             if (st == VS_ESCAPED) { fate = DSMC_FATE_ESCAPED; break; }
@@ -762,8 +857,11 @@ static void physical_particle(ctx_t *c, int64_t n, orng *rg)
             int ht = P->exp_grid ? exp_ht_index(Ri, R, c->unih, c->alpha2)
                                  : (int)ceil((Ri - R) / c->unih) - 1;
             if (ht < 0) ht = 0;
+            double nsig = 0.0;
             if (ht < hd) {
                 size_t cell = (size_t)ht * ftd * ffd + td * ffd + fd;
+                if (c->coll) nsig = c->nsig[cell];
+                visit_add(c, &vi, (int)cell, v);
                 #pragma omp atomic
                 c->density[cell] += W;
                 if (c->vmom) {
@@ -779,8 +877,21 @@ static void physical_particle(ctx_t *c, int64_t n, orng *rg)
                     c->vcount[cell] += 1;
                 }
             }
+            if (c->coll) {
+                double dx = s.x - so.x, dy = s.y - so.y, dz = s.z - so.z;
+                tau += nsig * sqrt(dx * dx + dy * dy + dz * dz);
+            }
             if (steps >= cap) { fate = DSMC_FATE_CAPPED; break; }
+            if (c->coll && tau >= tau_next) {
+                size_t cell = (size_t)ht * ftd * ffd + td * ffd + fd;
+                phys_collide(c, &v, cell, rg);
+                ncol++;
+                tau = 0.0;
+                tau_next = -OM_LOG(orng_u01(rg));
+                visit_flush(c, &vi);
+            }
         }
+        visit_flush(c, &vi);
         if (!landed) break;
         if (steps >= cap) { fate = DSMC_FATE_CAPPED; break; }
         int f = locate_face(c, s);
@@ -799,6 +910,10 @@ static void physical_particle(ctx_t *c, int64_t n, orng *rg)
         s.x *= sc2; s.y *= sc2; s.z *= sc2;
         v = emit(c, s, Tf, rg);
         hops++;
+    }
+    if (ncol) {
+        #pragma omp atomic
+        c->n_collisions += ncol;
     }
     end_particle(c, n, fate, steps, hops, s, v);
 }
@@ -926,6 +1041,13 @@ int cpu_ref_ballistic(const dsmc_params *p, const dsmc_surface *s, dsmc_result *
         c.sun_x = OM_SIN(p->subsolar_lon);
         c.sun_y = OM_COS(p->subsolar_lon);
         c.omega = 2.0 * M_PI / p->period;
+        if (r->vhist) {
+            if (p->vhist_bins <= 0 || p->vhist_bins > 64 || !(p->vhist_max > 0)) return DSMC_ERR_ARG;
+            memset(r->vhist, 0, cells * p->vhist_bins * sizeof(int64_t));
+            c.vhist = r->vhist;
+            c.vbins = p->vhist_bins;
+            c.vinv = (double)p->vhist_bins / p->vhist_max;
+        }
         run_particles(&c, physical_particle, nthreads, rng_mode, 67890u, 12345u);
         free(c.cdf);
     }
@@ -998,9 +1120,125 @@ void cpu_ref_bin_records(const double *vrec, const int32_t *vtd, const int32_t *
     }
 }
 
+/* Physical collisional fits from phase-1 speed histograms (cells [ht][td][fd]):
+ * P = 50 (steps in bins >= k) / (steps of the cell) at the occupied bins lo..hi,
+ * fitted in x = (k - lo + 0.5) / (hi - lo + 1) with the polyfit normal equations. */
+void cpu_ref_hist_fits(const dsmc_params *p, const int64_t *vhist, double *coeffs, double *pmin, double *pmax,
+                       int *valid)
+{
+    const int ncell = p->ftd * p->ffd * p->hd, bins = p->vhist_bins, deg = p->fit_degree;
+    const double dv = p->vhist_max / p->vhist_bins;
+    for (int cidx = 0; cidx < ncell; cidx++) {
+        const int64_t *h = vhist + (size_t)cidx * bins;
+        int64_t N = 0;
+        int lo = -1, hi = -1;
+        for (int b = 0; b < bins; b++)
+            if (h[b]) { N += h[b]; if (lo < 0) lo = b; hi = b; }
+        valid[cidx] = 0;
+        pmin[cidx] = pmax[cidx] = 0.0;
+        for (int i = 0; i <= deg; i++) coeffs[(size_t)cidx * (deg + 1) + i] = 0.0;
+        if (lo < 0 || N <= (p->min_fit_samples > 0 ? p->min_fit_samples : 0)) continue;
+        const int npts = hi - lo + 1;
+        if (npts < deg + 2) continue;
+        double x[64], y[64], cf[7];
+        int64_t acc = 0;
+        for (int k = hi; k >= lo; k--) {
+            acc += h[k];
+            x[k - lo] = ((double)(k - lo) + 0.5) / (double)npts;
+            y[k - lo] = 50.0 * (double)acc / (double)N;
+        }
+        if (cpu_ref_polyfit(x, y, npts, deg, cf) != 0) continue;
+        for (int i = 0; i <= deg; i++) coeffs[(size_t)cidx * (deg + 1) + i] = cf[i];
+        pmin[cidx] = (double)lo * dv;
+        pmax[cidx] = (double)(hi + 1) * dv;
+        valid[cidx] = 1;
+    }
+}
+
+/* Number density of a phase-1 run: vcount x dt x (sum of face weights x R^2 x
+ * sublimation_scale) / (nparticles x cell volume). */
+int cpu_ref_number_density(const dsmc_params *p1, const dsmc_surface *s, const int64_t *vcount, double *ndens)
+{
+    const int nfine = 20 << (2 * s->levels);
+    double F = 0.0;
+    for (int f = 0; f < nfine; f++) F += s->face_w[f];
+    const double R = p1->radius;
+    F = F * R * R * p1->sublimation_scale;
+    const int ftd = p1->ftd, ffd = p1->ffd, hd = p1->hd;
+    double *redge = malloc(sizeof(double) * (hd + 1)), *sl = malloc(sizeof(double) * (ftd + 1));
+    for (int k = 0; k <= hd; k++)
+        redge[k] = p1->exp_grid ? R + p1->unih * (pow(1.0 + p1->alpha2, k) - 1.0) / p1->alpha2 : R + p1->unih * k;
+    for (int t = 0; t <= ftd; t++) sl[t] = sin(-M_PI / 2.0 + M_PI * t / ftd);
+    const double dphi = 2.0 * M_PI / ffd, N = (double)p1->nparticles;
+    for (int ht = 0; ht < hd; ht++) {
+        const double r0 = redge[ht], r1 = redge[ht + 1];
+        const double shell = (r1 * r1 * r1 - r0 * r0 * r0) / 3.0;
+        for (int td = 0; td < ftd; td++) {
+            const double V = shell * (sl[td + 1] - sl[td]) * dphi;
+            for (int fd = 0; fd < ffd; fd++) {
+                const size_t cidx = ((size_t)ht * ftd + td) * ffd + fd;
+                ndens[cidx] = (double)vcount[cidx] * p1->dt * F / (N * V);
+            }
+        }
+    }
+    free(redge);
+    free(sl);
+    return DSMC_OK;
+}
+
+/* Physical phase 2: collisions against the phase-1 density and fits. */
+static int advanced_physical(const dsmc_params *p, const dsmc_surface *s, const dsmc_phase2_input *in,
+                             dsmc_result *r, int nthreads, int rng_mode)
+{
+    if (!in->ndens || !in->vhist || p->fit_degree < 1 || p->fit_degree > 6 || p->vhist_bins <= 0 ||
+        p->vhist_bins > 64 || !(p->vhist_max > 0))
+        return DSMC_ERR_ARG;
+    const size_t ncell = (size_t)p->ftd * p->ffd * p->hd;
+    double *coeffs = calloc(ncell * (p->fit_degree + 1), sizeof(double));
+    double *pmin = calloc(ncell, sizeof(double)), *pmax = calloc(ncell, sizeof(double));
+    double *nsig = calloc(ncell, sizeof(double));
+    int *valid = calloc(ncell, sizeof(int));
+    cpu_ref_hist_fits(p, in->vhist, coeffs, pmin, pmax, valid);
+    for (size_t i = 0; i < ncell; i++) nsig[i] = valid[i] ? in->ndens[i] * p->cross_section : 0.0;
+    ctx_t c;
+    ctx_common(&c, p, r);
+    double *den = r->density ? r->density : calloc(ncell, sizeof(double));
+    double *hor = r->horizon ? r->horizon : calloc((size_t)p->ftd * p->ffd, sizeof(double));
+    zero_result(p, r, 5);
+    memset(den, 0, ncell * sizeof(double));
+    memset(hor, 0, (size_t)p->ftd * p->ffd * sizeof(double));
+    c.density = den; c.horizon = hor; c.vmom = r->vmom; c.vcount = r->vcount;
+    c.surf = s;
+    int off = 0, nl = 20;
+    for (int l = 0; l <= s->levels; l++) {
+        c.lvl_faces[l] = s->faces + 3 * (size_t)off;
+        c.lvl_n[l] = nl;
+        off += nl;
+        nl *= 4;
+    }
+    c.nfine = c.lvl_n[s->levels];
+    c.cdf = malloc(sizeof(double) * c.nfine);
+    double acc = 0.0;
+    for (int i = 0; i < c.nfine; i++) { acc += s->face_w[i]; c.cdf[i] = acc; }
+    c.sun_x = OM_SIN(p->subsolar_lon);
+    c.sun_y = OM_COS(p->subsolar_lon);
+    c.omega = 2.0 * M_PI / p->period;
+    c.coll = 1;
+    c.nsig = nsig;
+    c.poly_coeffs = coeffs; c.poly_min = pmin; c.poly_max = pmax; c.poly_valid = valid;
+    run_particles(&c, physical_particle, nthreads, rng_mode, 67890u, 12345u);
+    ctx_finish(&c, r);
+    free(c.cdf); free(coeffs); free(pmin); free(pmax); free(nsig); free(valid);
+    if (!r->density) free(den);
+    if (!r->horizon) free(hor);
+    return DSMC_OK;
+}
+
 int cpu_ref_advanced(const dsmc_params *p, const dsmc_surface *s, const dsmc_phase2_input *in,
                      dsmc_result *r, int nthreads, int rng_mode)
 {
+    if (p && s && r && in && p->mode == DSMC_MODE_PHYSICAL)
+        return advanced_physical(p, s, in, r, nthreads, rng_mode);
     if (!p || !s || !r || !in || p->mode != DSMC_MODE_COMPAT || !s->T_latlon ||
         s->ftd != p->ftd || s->ffd != p->ffd || p->fit_degree < 1 || p->fit_degree > 6 ||
         p->max_per_cell <= 0 || p->max_per_cell > 256)
@@ -1045,6 +1283,33 @@ int cpu_ref_advanced(const dsmc_params *p, const dsmc_surface *s, const dsmc_pha
     if (!r->density) free(den);
     if (!r->horizon) free(hor);
     return DSMC_OK;
+}
+
+/* Phase 1 with histograms, the number density, phase 2 (physical collisional). */
+int cpu_ref_collisional(const dsmc_params *p1, const dsmc_params *p2, const dsmc_surface *s, dsmc_result *r1,
+                        dsmc_result *r2, int nthreads, int rng_mode)
+{
+    const size_t ncell = (size_t)p1->ftd * p1->ffd * p1->hd;
+    dsmc_result q;
+    if (r1) q = *r1;
+    else memset(&q, 0, sizeof q);
+    int64_t *cnt = q.vcount ? q.vcount : calloc(ncell, sizeof(int64_t));
+    int64_t *vh = q.vhist ? q.vhist : calloc(ncell * p1->vhist_bins, sizeof(int64_t));
+    q.vcount = cnt;
+    q.vhist = vh;
+    int rc = cpu_ref_ballistic(p1, s, &q, nthreads, rng_mode);
+    double *nd = calloc(ncell, sizeof(double));
+    if (!rc) rc = cpu_ref_number_density(p1, s, cnt, nd);
+    dsmc_phase2_input in;
+    memset(&in, 0, sizeof in);
+    in.ndens = nd;
+    in.vhist = vh;
+    if (!rc) rc = cpu_ref_advanced(p2, s, &in, r2, nthreads, rng_mode);
+    if (r1) { int64_t *a = r1->vcount, *b = r1->vhist; *r1 = q; r1->vcount = a; r1->vhist = b; }
+    if (!r1 || !r1->vcount) free(cnt);
+    if (!r1 || !r1->vhist) free(vh);
+    free(nd);
+    return rc;
 }
 
 /* Number of OpenMP threads this build would use by default. */

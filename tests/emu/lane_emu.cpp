@@ -49,7 +49,7 @@ static void run_one_split(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t 
             lane_step<DsmcOpsHost, MODE, ADV>(F, E, TB);
             if (F.phase == LANE_FLIGHT && F.steps >= end) { paused = true; break; }
         }
-        lane_flush<DsmcOpsHost>(F, E);
+        if (!paused || MODE == 0) lane_flush<DsmcOpsHost>(F, E);   /* physical pauses keep the visit */
         rec_store_flight<MODE, ADV>(F, R);
         if (paused) continue;
         DsmcLane S;
@@ -61,13 +61,14 @@ static void run_one_split(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t 
     }
 }
 
+template <int ADV>
 static void run_one_inline(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t budget)
 {
     DsmcRec R;
     memset(&R, 0x5A, sizeof R);
     DsmcLane L;
     garbage(L);
-    lane_start<DsmcOpsHost, 1, 0>(L, E, n, cnt);
+    lane_start<DsmcOpsHost, 1, ADV>(L, E, n, cnt);
     if (L.phase != LANE_FLIGHT) return;
     rec_store_all<1>(L, R, 0);
     DsmcCellTabs TB = {E.tq, E.hr, E.cs};
@@ -78,16 +79,15 @@ static void run_one_inline(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t
         const int64_t end = F.steps + budget;
         for (;;) {
             bool paused = false;
-            lane_step<DsmcOpsHost, 1, 0>(F, E, TB);
+            lane_step<DsmcOpsHost, 1, ADV>(F, E, TB);
             if (F.phase == LANE_FLIGHT && F.steps >= end) paused = true;
-            if (F.phase == LANE_LANDED || F.phase == LANE_ENDED) {
+            if (F.phase == LANE_LANDED || F.phase == LANE_ENDED || F.phase == LANE_COLLIDE) {
                 DsmcLane T = F;
-                lane_service<DsmcOpsHost, 1, 0>(T, E, cnt);
+                lane_service<DsmcOpsHost, 1, ADV>(T, E, cnt);
                 F = T;
                 if (F.phase == LANE_DONE) return;
             }
-            if (paused) {
-                lane_flush<DsmcOpsHost>(F, E);
+            if (paused) {                  /* the open visit travels with the record */
                 rec_store_phys(F, R);
                 break;
             }
@@ -166,6 +166,13 @@ extern "C" int emu_ballistic_path(const dsmc_params *p, const dsmc_surface *s, d
     E.end_state = r->end_state; E.steps_out = r->steps; E.fate_out = r->fate; E.hops_out = r->hops;
     E.vrec = r->vrec; E.vrec_td = r->vrec_td; E.vrec_fd = r->vrec_fd; E.vrec_ht = r->vrec_ht;
     E.count_nn = count_nn.data(); E.rec_total = &rec_total;
+    std::vector<unsigned long long> vh;
+    if (r->vhist && p->mode == DSMC_MODE_PHYSICAL) {
+        vh.assign(cells * p->vhist_bins, 0);
+        E.vhist = vh.data();
+        E.vhist_bins = p->vhist_bins;
+        E.vhist_inv = (double)p->vhist_bins / p->vhist_max;
+    }
     DsmcCounts cnt;
     memset(&cnt, 0, sizeof cnt);
     for (int64_t n = 0; n < p->nparticles; n++) {
@@ -173,13 +180,14 @@ extern "C" int emu_ballistic_path(const dsmc_params *p, const dsmc_surface *s, d
         memset(&L, 0, sizeof L);
         if (path == 1 && p->mode == DSMC_MODE_COMPAT) run_one_split<0, 0>(E, n, cnt, budget);
         else if (path == 1) run_one_split<1, 0>(E, n, cnt, budget);
-        else if (path == 2) run_one_inline(E, n, cnt, budget);
+        else if (path == 2) run_one_inline<0>(E, n, cnt, budget);
         else if (p->mode == DSMC_MODE_COMPAT) run_one<0, 0>(L, E, n, cnt);
         else run_one<1, 0>(L, E, n, cnt);
     }
     if (r->density) memcpy(r->density, den.data(), cells * sizeof(double));
     if (r->horizon) memcpy(r->horizon, hor.data(), hor.size() * sizeof(double));
     if (r->vcount) for (size_t i = 0; i < cells; i++) r->vcount[i] = (int64_t)vcount[i];
+    if (E.vhist) for (size_t i = 0; i < vh.size(); i++) r->vhist[i] = (int64_t)vh[i];
     r->vrec_n = (int64_t)(rec_total < (unsigned long long)p->record_cap_total ? rec_total : p->record_cap_total);
     r->n_escaped = cnt.esc; r->n_frozen = cnt.frozen; r->n_photolysed = cnt.photo;
     r->n_depleted = cnt.depleted; r->n_capped = cnt.capped; r->n_collisions = cnt.coll;
@@ -190,4 +198,46 @@ extern "C" int emu_ballistic_path(const dsmc_params *p, const dsmc_surface *s, d
 extern "C" int emu_ballistic(const dsmc_params *p, const dsmc_surface *s, dsmc_result *r)
 {
     return emu_ballistic_path(p, s, r, 0, 0);
+}
+
+/* Physical collisional phase 2 on the lane state machine with the caller's
+ * fits and n*sigma table: path 0 straight, 2 the flight kernel with the service
+ * inline (budget steps per launch). */
+extern "C" int emu_physical_coll(const dsmc_params *p, const dsmc_surface *s, const double *nsig,
+                                 const double *coeffs, const double *pmin, const double *pmax, dsmc_result *r,
+                                 int path, int64_t budget)
+{
+    if (p->mode != DSMC_MODE_PHYSICAL || (path != 0 && path != 2) || (path == 2 && budget < 1)) return -1;
+    DsmcEnv E;
+    dsmc::HostTables T;
+    int rc = dsmc::build_env(p, s, 1, &E, &T);
+    if (rc) return rc;
+    size_t cells = (size_t)p->hd * p->ftd * p->ffd;
+    std::vector<double> den(cells, 0.0), hor((size_t)p->ftd * p->ffd, 0.0);
+    std::vector<unsigned long long> vcount(cells, 0);
+    E.tq = T.tq.data(); E.hr = T.hr.data(); E.cs = T.cs.data();
+    E.cdf = T.cdf.data(); E.face_T = s->face_T; E.vert = s->vert; E.faces = s->faces;
+    E.enorm = T.enorm.data(); E.fnorm = T.fnorm.data();
+    E.density = den.data(); E.horizon = hor.data();
+    E.vmom = r->vmom;
+    if (r->vmom) memset(r->vmom, 0, cells * 3 * sizeof(double));
+    E.vcount = vcount.data();
+    E.density_from_count = 1;
+    E.end_state = r->end_state; E.steps_out = r->steps; E.fate_out = r->fate; E.hops_out = r->hops;
+    E.nsig = nsig; E.coeffs = coeffs; E.pmin = pmin; E.pmax = pmax; E.fit_deg = p->fit_degree;
+    DsmcCounts cnt;
+    memset(&cnt, 0, sizeof cnt);
+    for (int64_t n = 0; n < p->nparticles; n++) {
+        DsmcLane L;
+        memset(&L, 0, sizeof L);
+        if (path == 2) run_one_inline<1>(E, n, cnt, budget);
+        else run_one<1, 1>(L, E, n, cnt);
+    }
+    if (r->density) for (size_t i = 0; i < cells; i++) r->density[i] = (double)vcount[i];
+    if (r->horizon) memcpy(r->horizon, hor.data(), hor.size() * sizeof(double));
+    if (r->vcount) for (size_t i = 0; i < cells; i++) r->vcount[i] = (int64_t)vcount[i];
+    r->n_escaped = cnt.esc; r->n_frozen = cnt.frozen; r->n_photolysed = cnt.photo;
+    r->n_depleted = cnt.depleted; r->n_capped = cnt.capped; r->n_collisions = cnt.coll;
+    r->total_steps = cnt.steps;
+    return 0;
 }
