@@ -179,19 +179,50 @@ def batched_run(lib, args, shape) -> dict:
     return out
 
 
+def collisional_run(lib, args) -> dict:
+    """The collisional workload: Ceres H2O, phase 1 (collisionless, speed
+    histograms) then phase 2 (hard spheres d = 4.6e-10 m, dt 5 s, degree-6
+    partner fits), sublimation x10, grid 50x36x72, both phases at
+    args.coll_particles, through the device-resident two-phase pipeline."""
+    surf = icosphere(lib, 3)
+    n = args.coll_particles
+    p1 = capi.preset(lib, capi.PRESET_CERES_PHYSICAL, 36, 72, n=n, seed=args.seed)
+    p1.sublimation_scale = 10.0
+    p2 = capi.preset(lib, capi.PRESET_CERES_PHYSICAL_COLL, 36, 72, n=n, seed=args.seed)
+    r1 = Buffers(p1, 5, per_particle=False, moments=True, vhist=True)
+    r2 = Buffers(p2, 5, per_particle=False, moments=True)
+    t0 = time.perf_counter()
+    rc = lib.dsmc_run_collisional(C.byref(p1), C.byref(p2), C.byref(surf.surface), C.byref(r1.result),
+                                  C.byref(r2.result))
+    wall = time.perf_counter() - t0
+    if rc:
+        raise RuntimeError(f"dsmc_run_collisional: {rc}")
+    s1, s2 = int(r1.result.total_steps), int(r2.result.total_steps)
+    return {"value": (s1 + s2) / wall, "unit": "particle-steps/s", "particles_per_phase": n,
+            "wall_ms": wall * 1e3, "phase1_steps": s1, "phase2_steps": s2,
+            "collisions": int(r2.result.n_collisions), "collisions_per_phase2_step": r2.result.n_collisions / max(s2, 1),
+            "phase2_fates": {k: int(getattr(r2.result, k)) for k in
+                             ("n_escaped", "n_frozen", "n_photolysed", "n_capped")}}
+
+
 def served_run(lib, args, shape) -> dict:
-    """Poisson request stream (20 req/s) of ballistic runs, counts log-uniform on
-    [1e4, 4e6], body and subsolar longitude drawn per request, through the
-    continuous-batching server; latency = arrival to result ready."""
-    stream = W.request_stream(args.serve_rate, args.serve_seconds, seed=5, scale=args.serve_scale)
-    skipped = sum(1 for _, s in stream if s.collisional)
-    stream = [(t, s) for t, s in stream if not s.collisional]
+    """Poisson request stream (20 req/s) of runs, counts log-uniform on [1e4,
+    4e6], body and subsolar longitude drawn per request, 30% collisional (phase 1
+    + phase 2 at the request's count), through the continuous-batching server;
+    latency = arrival to result ready."""
+    stream = W.request_stream(args.serve_rate, args.serve_seconds, seed=5, scale=args.serve_scale,
+                              frac_collisional=args.serve_collisional)
     specs = [s for _, s in stream]
     params, surfs = W.build_runs(lib, specs, levels=4, hd=40)
-    sv = lib.dsmc_server_create(args.serve_budget_gb << 30, 256, 0)
+    p2s = [None] * len(specs)
+    for i, sp in enumerate(specs):
+        if sp.collisional:
+            params[i], p2s[i] = W.coll_params(lib, sp.body, sp.n, sp.lon, sp.seed, hd=40)
+    sv = lib.dsmc_server_create(args.serve_budget_gb << 30, 256, 2)      # collisional requests accepted
     if not sv:
         raise RuntimeError("dsmc_server_create failed")
-    bufs = [Buffers(p, 5, per_particle=False, moments=True) for p in params]
+    bufs = [Buffers(p2s[i] if p2s[i] is not None else p, 5, per_particle=False, moments=True)
+            for i, p in enumerate(params)]
     tickets, t_sub = [0] * len(stream), [0.0] * len(stream)
     t0 = time.perf_counter()
     i, outstanding = 0, set()
@@ -200,8 +231,13 @@ def served_run(lib, args, shape) -> dict:
         now = time.perf_counter() - t0
         while i < len(stream) and stream[i][0] <= now:
             tk = C.c_uint64()
-            rc = lib.dsmc_server_submit(sv, C.byref(params[i]), C.byref(surfs[i].surface),
-                                        C.byref(bufs[i].result), C.byref(tk))
+            if p2s[i] is not None:
+                rc = lib.dsmc_server_submit_collisional(sv, C.byref(params[i]), C.byref(p2s[i]),
+                                                        C.byref(surfs[i].surface), C.byref(bufs[i].result),
+                                                        C.byref(tk))
+            else:
+                rc = lib.dsmc_server_submit(sv, C.byref(params[i]), C.byref(surfs[i].surface),
+                                            C.byref(bufs[i].result), C.byref(tk))
             if rc:
                 raise RuntimeError(f"submit failed: {rc}")
             if first:
@@ -227,9 +263,10 @@ def served_run(lib, args, shape) -> dict:
     lib.dsmc_server_get_stats(sv, C.byref(stats))
     lib.dsmc_server_destroy(sv)
     lat = [bufs[j].result.elapsed_ms + (t_sub[j] - stream[j][0]) * 1e3 for j in range(len(stream))]
-    steps = sum(int(b.result.total_steps) for b in bufs)
+    steps = int(stats.particle_steps)
+    ncoll = sum(1 for s in specs if s.collisional)
     return {"value": steps / t_end, "unit": "particle-steps/s", "requests": len(stream),
-            "collisional_requests_not_served": skipped, "offered_seconds": args.serve_seconds,
+            "collisional_requests": ncoll, "offered_seconds": args.serve_seconds,
             "rate_per_s": args.serve_rate, "wall_s": t_end, "particles": int(sum(s.n for s in specs)),
             "particle_steps": steps, "latency": percentiles(lat), "rounds": int(stats.rounds),
             "slots": int(stats.slots)}
@@ -260,6 +297,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-scale", type=float, default=1.0)
     ap.add_argument("--serve-budget-gb", type=int, default=128)
+    ap.add_argument("--serve-collisional", type=float, default=0.3, help="fraction of collisional requests")
+    ap.add_argument("--no-coll", action="store_true")
+    ap.add_argument("--coll-particles", type=int, default=2_000_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -299,6 +339,8 @@ def main():
         tuned = capi.LaunchShape(s["block"], s["blocks_per_cu"], s["relaunch_batch"], s["step_budget"],
                                  s["service_inline"])
         line["batched_vs_plain"] = bt["value"] / main_run["value"]
+    if not args.no_coll:
+        line["collisional"] = collisional_run(lib, args)
     if not args.no_serve:
         line["served"] = served_run(lib, args, shape if shape is not None else tuned)
     if not args.no_cpu:

@@ -196,6 +196,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.dsmc_server_create.restype = C.c_void_p
     lib.dsmc_server_set_shape.argtypes = [C.c_void_p, C.POINTER(LaunchShape)]
     lib.dsmc_server_submit.argtypes = [C.c_void_p, P, S, R, C.POINTER(C.c_uint64)]
+    lib.dsmc_server_submit_collisional.argtypes = [C.c_void_p, P, P, S, R, C.POINTER(C.c_uint64)]
     lib.dsmc_server_poll.argtypes = [C.c_void_p, C.c_uint64]
     lib.dsmc_server_wait.argtypes = [C.c_void_p, C.c_uint64]
     lib.dsmc_server_cancel.argtypes = [C.c_void_p, C.c_uint64]

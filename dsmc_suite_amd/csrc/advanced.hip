@@ -225,6 +225,22 @@ __global__ void nsig_kernel(const double *ndens, const int32_t *valid, double si
     nsig[c] = valid[c] ? ndens[c] * sigma : 0.0;
 }
 
+extern "C" hipError_t dsmc_launch_hist_fits(const unsigned long long *vhist, int ncell, int bins, double dv, int degree,
+                                            int min_samples, double *coeffs, double *pmin, double *pmax,
+                                            int32_t *valid, int32_t *count_num, hipStream_t st)
+{
+    hipLaunchKernelGGL(hist_fit_kernel, dim3((ncell + 63) / 64), dim3(64), 0, st, vhist, ncell, bins, dv, degree,
+                       min_samples, coeffs, pmin, pmax, valid, count_num);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t dsmc_launch_nsig(const double *ndens, const int32_t *valid, double sigma, int ncell, double *nsig,
+                                       hipStream_t st)
+{
+    hipLaunchKernelGGL(nsig_kernel, dim3((ncell + 255) / 256), dim3(256), 0, st, ndens, valid, sigma, ncell, nsig);
+    return hipGetLastError();
+}
+
 using namespace dsmc;
 
 /* Physical phase-2 tables: fits from the histograms (vhist_dev: device array,
@@ -440,8 +456,16 @@ extern "C" int dsmc_number_density(const dsmc_params *p1, const dsmc_surface *s,
     const int nfine = 20 << (2 * s->levels);
     double F = 0.0;
     for (int f = 0; f < nfine; f++) F += s->face_w[f];
+    number_density(p1, F, vcount, ndens);
+    return DSMC_OK;
+}
+
+namespace dsmc {
+/* ndens from the sum of the face weights (in face order). */
+void number_density(const dsmc_params *p1, double wsum, const int64_t *vcount, double *ndens)
+{
     const double R = p1->radius;
-    F = F * R * R * p1->sublimation_scale;            /* molecules per second */
+    const double F = wsum * R * R * p1->sublimation_scale;   /* molecules per second */
     const int ftd = p1->ftd, ffd = p1->ffd, hd = p1->hd;
     std::vector<double> redge(hd + 1), sl(ftd + 1);
     for (int k = 0; k <= hd; k++)
@@ -459,8 +483,8 @@ extern "C" int dsmc_number_density(const dsmc_params *p1, const dsmc_surface *s,
             }
         }
     }
-    return DSMC_OK;
 }
+}  // namespace dsmc
 
 extern "C" int dsmc_run_collisional(const dsmc_params *p1, const dsmc_params *p2, const dsmc_surface *s,
                                     dsmc_result *r1, dsmc_result *r2)

@@ -48,6 +48,7 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
 int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int timed = 0);
 int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st);
 dsmc_launch_shape default_shape(const DsmcEnv &E);
+void number_density(const dsmc_params *p1, double wsum, const int64_t *vcount, double *ndens);
 bool shape_ok(const dsmc_launch_shape &s);
 
 }  // namespace dsmc

@@ -44,6 +44,13 @@ hipError_t dsmc_launch_pool_service(const DsmcPoolDev *P, DsmcQueues Q, int p, i
  * one run (E on the host, Ed its copy in device memory, P == NULL) or a pool (P) */
 hipError_t dsmc_launch_phys(const DsmcEnv *E, const DsmcEnv *Ed, const DsmcPoolDev *P, DsmcRec *rec, DsmcQueues Q,
                             int p, int grid, int block, int svc_batch, int64_t step_budget, hipStream_t st);
+/* physical collisional tables: partner-speed fits from speed histograms, and
+ * n*sigma (ndens may alias nsig) */
+hipError_t dsmc_launch_hist_fits(const unsigned long long *vhist, int ncell, int bins, double dv, int degree,
+                                 int min_samples, double *coeffs, double *pmin, double *pmax, int32_t *valid,
+                                 int32_t *count_num, hipStream_t st);
+hipError_t dsmc_launch_nsig(const double *ndens, const int32_t *valid, double sigma, int ncell, double *nsig,
+                            hipStream_t st);
 /* n zeros of 8 bytes (graph-safe clear) */
 hipError_t dsmc_launch_fill0(void *p, size_t n8, hipStream_t st);
 hipError_t dsmc_launch_verlet(double *state, int n, int nsteps, double GM, double R,

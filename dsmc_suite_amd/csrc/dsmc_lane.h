@@ -121,6 +121,7 @@ struct DsmcStepEnv {
     int hr_off;               /* this run's radial thresholds within the pool's table */
     int pad_;
     const double *nsig;       /* collisional runs: n*sigma per cell */
+    const DsmcEnv *env;       /* the run's full environment (device memory) */
 };
 
 struct DsmcLane {
@@ -168,7 +169,10 @@ DSMC_HD unsigned long long *vhist_bin(const DsmcEnv &E, int cell, double vx, dou
     int bin = b < (double)(E.vhist_bins - 1) ? (int)b : E.vhist_bins - 1;   /* NaN -> last bin */
     return E.vhist + (size_t)cell * E.vhist_bins + bin;
 }
-DSMC_HD unsigned long long *vhist_bin(const DsmcStepEnv &, int, double, double, double, int) { return nullptr; }
+DSMC_HD unsigned long long *vhist_bin(const DsmcStepEnv &S, int cell, double vx, double vy, double vz, int n)
+{
+    return S.env ? vhist_bin(*S.env, cell, vx, vy, vz, n) : nullptr;
+}
 
 /* Atomic/plain accumulation, supplied by the caller (GPU atomics or host adds). */
 struct DsmcOpsHost {

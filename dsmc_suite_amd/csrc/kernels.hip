@@ -616,12 +616,16 @@ extern "C" hipError_t dsmc_launch_phys(const DsmcEnv *E, const DsmcEnv *Ed, cons
 {
     if (!block_ok(block)) return hipErrorInvalidValue;
     if (P) {
-        if (P->mode != 1 || P->adv) return hipErrorInvalidValue;
+        if (P->mode != 1) return hipErrorInvalidValue;
         const size_t lds = sizeof(double) * (size_t)((P->ftd + 1) + 2 * (P->ffd + 1) + P->nhr);
         DsmcEnv none;
         memset(&none, 0, sizeof none);
-        hipLaunchKernelGGL((phys_kernel<true, 0>), dim3(grid), dim3(block), lds, st, none, (const DsmcEnv *)nullptr, *P,
-                           rec, Q, p, svc_batch, step_budget);
+        if (P->adv)
+            hipLaunchKernelGGL((phys_kernel<true, 1>), dim3(grid), dim3(block), lds, st, none, (const DsmcEnv *)nullptr,
+                               *P, rec, Q, p, svc_batch, step_budget);
+        else
+            hipLaunchKernelGGL((phys_kernel<true, 0>), dim3(grid), dim3(block), lds, st, none, (const DsmcEnv *)nullptr,
+                               *P, rec, Q, p, svc_batch, step_budget);
     } else {
         if (E->mode != 1 || !Ed) return hipErrorInvalidValue;
         const size_t lds = sizeof(double) * (size_t)((E->ftd + 1) + 2 * (E->ffd + 1) + (E->hd + 2));

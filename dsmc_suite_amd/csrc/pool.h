@@ -19,6 +19,8 @@ struct PoolClass {
 
 struct PoolRun {
     int used = 0, done = 0, cancelled = 0;
+    int kind = 0;              /* 0: ballistic (optionally keeping histograms), 1: collisional */
+    int hist = -1, coll = -1;  /* histogram slot (phase 1) / fit-table slot (phase 2) */
     dsmc_params p;
     DsmcEnv E;
     int64_t slot0 = 0, n = 0;
@@ -46,6 +48,13 @@ struct Pool {
     int cus = 256;
 
     DevBuf rec, flyq0, flyq1, svcq, qctl;
+    DevBuf cflyq0, cflyq1, cqctl;              /* queues of collisional particles */
+    /* physical collisional requests: phase-1 speed histograms and phase-2 fit
+     * tables, POOL_MAX_HIST / POOL_MAX_COLL slots (allocated when enabled) */
+    int hist_bins = 0, fit_deg = 0;
+    DevBuf hist;                               /* [max_hist][cells*hist_bins] */
+    DevBuf coll_cf, coll_lo, coll_hi, coll_nsig, coll_valid, coll_cnt;
+    std::vector<int> hist_used, coll_used;
     DevBuf envs_d, steps_d, tq, cs, hr;
     DevBuf geo_vert, geo_faces, geo_enorm, geo_fnorm;   /* [max_geo] slots */
     DevBuf cdf, face_T;                        /* [max_runs][nfine] */
@@ -74,7 +83,7 @@ struct Pool {
     ~Pool();
 };
 
-enum { POOL_MAX_HR_TABS = 16, POOL_MAX_GEO = 4 };
+enum { POOL_MAX_HR_TABS = 16, POOL_MAX_GEO = 4, POOL_MAX_HIST = 8, POOL_MAX_COLL = 8 };
 
 /* Footprint in device memory of one particle slot and of one run's grid. */
 size_t pool_slot_bytes(int per_particle);
@@ -82,6 +91,7 @@ size_t pool_run_bytes(const PoolClass &c);
 
 /* A run prepared on the host (environment and tables), ready to admit. */
 struct PoolPrep {
+    int kind = 0;              /* 1: collisional phase 2 */
     dsmc_params p;
     DsmcEnv E;
     HostTables T;
@@ -92,12 +102,20 @@ struct PoolPrep {
 };
 
 int pool_class_of(const dsmc_params *p, const dsmc_surface *s, PoolClass *c);
-int pool_prepare(const dsmc_params *p, const dsmc_surface *s, PoolPrep *out);
+int pool_prepare(const dsmc_params *p, const dsmc_surface *s, PoolPrep *out, int collisional = 0);
+/* Histogram and fit-table slots for collisional requests (bins <= 64, degree <= 6). */
+int pool_enable_collisional(Pool *P, int bins, int degree);
+int pool_take_slot(std::vector<int> &used);    /* -1 if none free */
 int pool_create(Pool *P, const PoolClass &c, int max_runs, int64_t max_slots, int per_particle,
                 const dsmc_launch_shape *shape);
 /* Admits a run: DSMC_OK (*g = its run slot), DSMC_ERR_BUSY if slots, grid slots
  * or table slots are not free now, or another status. */
-int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g);
+/* hist: histogram slot for a ballistic run that keeps them (or -1); coll: fit-
+ * table slot of a collisional run (required for kind 1). */
+int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g, int hist = -1, int coll = -1);
+/* Builds fit table `coll` from histogram slot `hist` and the number density
+ * ndens (host array, [cells]) on the pool's stream; parameters of the phase-2 run. */
+int pool_build_fits(Pool *P, int hist, int coll, const double *ndens, const dsmc_params *p2);
 /* Enqueues `chunks` graph replays of the pool's rounds (at most 2 unread). */
 int pool_advance(Pool *P, int chunks);
 /* Waits for the oldest unread replay, reads the run counters it left, marks

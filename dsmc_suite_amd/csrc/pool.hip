@@ -45,7 +45,7 @@ static const int POOL_RING = 2;
 
 size_t pool_slot_bytes(int per_particle)
 {
-    return sizeof(DsmcRec) + 3 * sizeof(int32_t) + (per_particle ? 6 * 8 + 8 + 4 + 4 : 0);
+    return sizeof(DsmcRec) + 5 * sizeof(int32_t) + (per_particle ? 6 * 8 + 8 + 4 + 4 : 0);
 }
 
 size_t pool_run_bytes(const PoolClass &c)
@@ -98,13 +98,14 @@ static uint64_t fnv(const void *data, size_t n, uint64_t h = 1469598103934665603
     return h;
 }
 
-static DsmcQueues queues(Pool *P)
+/* Slot queues of ballistic (kind 0) or collisional (kind 1) particles. */
+static DsmcQueues queues(Pool *P, int kind = 0)
 {
     DsmcQueues Q;
-    Q.fly[0] = (int32_t *)P->flyq0.ptr;
-    Q.fly[1] = (int32_t *)P->flyq1.ptr;
+    Q.fly[0] = (int32_t *)(kind ? P->cflyq0.ptr : P->flyq0.ptr);
+    Q.fly[1] = (int32_t *)(kind ? P->cflyq1.ptr : P->flyq1.ptr);
     Q.svc = (int32_t *)P->svcq.ptr;
-    Q.ctl = (unsigned int *)P->qctl.ptr;
+    Q.ctl = (unsigned int *)(kind ? P->cqctl.ptr : P->qctl.ptr);
     return Q;
 }
 
@@ -133,8 +134,15 @@ static int launch_rounds(Pool *P, hipStream_t s)
     DsmcQueues Q = queues(P);
     int fgrid = P->cus * P->shape.blocks_per_cu;
     int sgrid = P->cus * 8;
+    DsmcPoolDev D1 = D;
+    D1.adv = 1;
+    DsmcQueues Q1 = queues(P, 1);
     for (int r = 0; r < P->graph_rounds; r++) {
         int p = r & 1;
+        if (P->hist_bins) {        /* collisional particles: their own queues and kernel */
+            HIP_OK(dsmc_launch_phys(nullptr, nullptr, &D1, D1.rec, Q1, p, fgrid, P->shape.block,
+                                    P->shape.relaunch_batch, P->shape.step_budget, s));
+        }
         if (P->shape.service_inline) {
             HIP_OK(dsmc_launch_phys(nullptr, nullptr, &D, D.rec, Q, p, fgrid, P->shape.block,
                                     P->shape.relaunch_batch, P->shape.step_budget, s));
@@ -149,6 +157,7 @@ static int launch_rounds(Pool *P, hipStream_t s)
 
 static int capture(Pool *P)
 {
+    if (P->st) HIP_OK(hipStreamSynchronize(P->st));   /* no replay of the old graph in flight */
     if (P->exec) { (void)hipGraphExecDestroy(P->exec); P->exec = nullptr; }
     if (!P->use_graph) return DSMC_OK;
     hipStream_t cap;
@@ -203,7 +212,9 @@ int pool_create(Pool *P, const PoolClass &c, int max_runs, int64_t max_slots, in
     const size_t ms = (size_t)max_slots, mr = (size_t)max_runs;
     if ((rc = P->rec.alloc(ms * sizeof(DsmcRec)))) return rc;
     if ((rc = P->flyq0.alloc(ms * 4)) || (rc = P->flyq1.alloc(ms * 4)) || (rc = P->svcq.alloc(ms * 4))) return rc;
+    if ((rc = P->cflyq0.alloc(ms * 4)) || (rc = P->cflyq1.alloc(ms * 4))) return rc;
     if ((rc = P->qctl.zeros(16 * sizeof(unsigned int), st))) return rc;
+    if ((rc = P->cqctl.zeros(16 * sizeof(unsigned int), st))) return rc;
     if ((rc = P->envs_d.zeros(mr * sizeof(DsmcEnv), st))) return rc;
     if ((rc = P->steps_d.zeros(mr * sizeof(DsmcStepEnv), st))) return rc;
     if ((rc = P->hr.zeros((size_t)POOL_MAX_HR_TABS * (c.hd + 2) * 8, st))) return rc;
@@ -295,13 +306,60 @@ static void give_slots(Pool *P, int64_t s0, int64_t n)
     P->free_slots[s0] = n;
 }
 
-int pool_prepare(const dsmc_params *p, const dsmc_surface *s, PoolPrep *out)
+int pool_take_slot(std::vector<int> &used)
+{
+    for (size_t i = 0; i < used.size(); i++)
+        if (!used[i]) { used[i] = 1; return (int)i; }
+    return -1;
+}
+
+int pool_enable_collisional(Pool *P, int bins, int degree)
+{
+    if (bins <= 0 || bins > 64 || degree < 1 || degree > 6) return DSMC_ERR_ARG;
+    if (P->hist_bins) return (bins == P->hist_bins && degree == P->fit_deg) ? DSMC_OK : DSMC_ERR_ARG;
+    const size_t c = P->cells;
+    int rc;
+    if ((rc = P->hist.alloc((size_t)POOL_MAX_HIST * c * bins * 8))) return rc;
+    if ((rc = P->coll_cf.alloc((size_t)POOL_MAX_COLL * c * (degree + 1) * 8)) ||
+        (rc = P->coll_lo.alloc((size_t)POOL_MAX_COLL * c * 8)) || (rc = P->coll_hi.alloc((size_t)POOL_MAX_COLL * c * 8)) ||
+        (rc = P->coll_nsig.alloc((size_t)POOL_MAX_COLL * c * 8)) ||
+        (rc = P->coll_valid.alloc((size_t)POOL_MAX_COLL * c * 4)) || (rc = P->coll_cnt.alloc((size_t)POOL_MAX_COLL * c * 4)))
+        return rc;
+    P->hist_used.assign(POOL_MAX_HIST, 0);
+    P->coll_used.assign(POOL_MAX_COLL, 0);
+    P->hist_bins = bins;
+    P->fit_deg = degree;
+    return capture(P);          /* the rounds now include the collisional kernel */
+}
+
+int pool_build_fits(Pool *P, int hist, int coll, const double *ndens, const dsmc_params *p2)
+{
+    if (!P->hist_bins || hist < 0 || hist >= POOL_MAX_HIST || coll < 0 || coll >= POOL_MAX_COLL ||
+        p2->vhist_bins != P->hist_bins || p2->fit_degree != P->fit_deg)
+        return DSMC_ERR_ARG;
+    const size_t c = P->cells;
+    const int deg = P->fit_deg;
+    double *nsig = (double *)P->coll_nsig.ptr + (size_t)coll * c;
+    int32_t *valid = (int32_t *)P->coll_valid.ptr + (size_t)coll * c;
+    /* the slot is not read by queued work (its run is admitted afterwards) */
+    HIP_OK(hipMemcpy(nsig, ndens, c * 8, hipMemcpyHostToDevice));
+    HIP_OK(dsmc_launch_hist_fits((const unsigned long long *)P->hist.ptr + (size_t)hist * c * P->hist_bins, (int)c,
+                                 P->hist_bins, p2->vhist_max / p2->vhist_bins, deg, p2->min_fit_samples,
+                                 (double *)P->coll_cf.ptr + (size_t)coll * c * (deg + 1),
+                                 (double *)P->coll_lo.ptr + (size_t)coll * c, (double *)P->coll_hi.ptr + (size_t)coll * c,
+                                 valid, (int32_t *)P->coll_cnt.ptr + (size_t)coll * c, P->st));
+    HIP_OK(dsmc_launch_nsig(nsig, valid, p2->cross_section, (int)c, nsig, P->st));
+    return DSMC_OK;
+}
+
+int pool_prepare(const dsmc_params *p, const dsmc_surface *s, PoolPrep *out, int collisional)
 {
     PoolClass c;
     int rc = pool_class_of(p, s, &c);
     if (rc) return rc;
     if (p->nparticles <= 0 || p->nparticles > INT32_MAX) return DSMC_ERR_ARG;
-    if ((rc = build_env(p, s, 0, &out->E, &out->T))) return rc;
+    if ((rc = build_env(p, s, collisional ? 1 : 0, &out->E, &out->T))) return rc;
+    out->kind = collisional ? 1 : 0;
     out->p = *p;
     const size_t nf = (size_t)out->T.nfine, nfa = (size_t)out->T.nface_all;
     out->face_T.assign(s->face_T, s->face_T + nf);
@@ -312,12 +370,16 @@ int pool_prepare(const dsmc_params *p, const dsmc_surface *s, PoolPrep *out)
     return DSMC_OK;
 }
 
-int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out)
+int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out, int hist, int coll)
 {
     const dsmc_params *p = &pr.p;
     const HostTables &T = pr.T;
     if (p->mode != P->cls.mode || p->ftd != P->cls.ftd || p->ffd != P->cls.ffd || p->hd != P->cls.hd ||
         T.levels != P->cls.levels || pr.vert.size() != P->nvert * 3)
+        return DSMC_ERR_ARG;
+    if ((pr.kind == 1) != (coll >= 0) || (hist >= 0 && pr.kind == 1)) return DSMC_ERR_ARG;
+    if ((hist >= 0 || coll >= 0) && (!P->hist_bins || hist >= POOL_MAX_HIST || coll >= POOL_MAX_COLL ||
+                                     p->vhist_bins != P->hist_bins || p->fit_degree != P->fit_deg))
         return DSMC_ERR_ARG;
     if (p->nparticles > P->max_slots) return DSMC_ERR_BUDGET;
     int g = -1;
@@ -409,6 +471,23 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out)
         E.fate_out = (int32_t *)P->fate.ptr + s0;
         E.hops_out = (int32_t *)P->hops.ptr + s0;
     }
+    E.vhist = nullptr;
+    E.nsig = nullptr;
+    if (hist >= 0) {
+        E.vhist = (unsigned long long *)P->hist.ptr + (size_t)hist * cells * P->hist_bins;
+        E.vhist_bins = P->hist_bins;
+        E.vhist_inv = (double)P->hist_bins / p->vhist_max;
+    }
+    if (coll >= 0) {
+        const int deg = P->fit_deg;
+        E.coeffs = (const double *)P->coll_cf.ptr + (size_t)coll * cells * (deg + 1);
+        E.pmin = (const double *)P->coll_lo.ptr + (size_t)coll * cells;
+        E.pmax = (const double *)P->coll_hi.ptr + (size_t)coll * cells;
+        E.pvalid = (const int32_t *)P->coll_valid.ptr + (size_t)coll * cells;
+        E.count_num = (const int32_t *)P->coll_cnt.ptr + (size_t)coll * cells;
+        E.nsig = (const double *)P->coll_nsig.ptr + (size_t)coll * cells;
+        E.fit_deg = deg;
+    }
     DsmcStepEnv S;
     memset(&S, 0, sizeof S);
     S.h = E.h; S.GM = E.GM; S.c1 = E.c1; S.c2 = E.c2; S.R = E.R; S.max_height = E.max_height;
@@ -416,6 +495,8 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out)
     S.ftd = E.ftd; S.ffd = E.ffd; S.hd = E.hd; S.density_from_count = 1;
     S.density = nullptr; S.vmom = E.vmom; S.vcount = E.vcount;
     S.hr_off = tab * hd2;
+    S.nsig = E.nsig;
+    S.env = hist >= 0 ? (const DsmcEnv *)P->envs_d.ptr + g : nullptr;
     P->envs_h[g] = E;
     P->steps_h[g] = S;
     HIP_OK(hipMemcpyAsync((DsmcEnv *)P->envs_d.ptr + g, &P->envs_h[g], sizeof(DsmcEnv), hipMemcpyHostToDevice, st));
@@ -425,6 +506,7 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out)
     HIP_OK(dsmc_launch_fill0(E.vmom, cells * 3, st));
     HIP_OK(dsmc_launch_fill0(E.horizon, (size_t)P->cls.ftd * P->cls.ffd, st));
     HIP_OK(dsmc_launch_fill0(E.counters, 8, st));
+    if (E.vhist) HIP_OK(dsmc_launch_fill0(E.vhist, cells * P->hist_bins, st));
 
     R.used = 1;
     R.done = 0;
@@ -434,10 +516,13 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out)
     R.hr_tab = tab;
     R.geo = geo;
     R.tag = tag;
+    R.kind = pr.kind;
+    R.hist = hist;
+    R.coll = coll;
     R.first_seq = P->issued;
     int64_t blocks = (R.n + 255) / 256;
     int grid = (int)std::min<int64_t>(blocks, (int64_t)P->cus * 8);
-    HIP_OK(dsmc_launch_start(&E, g, (DsmcRec *)P->rec.ptr, s0, R.n, queues(P), 0, grid, st));
+    HIP_OK(dsmc_launch_start(&E, g, (DsmcRec *)P->rec.ptr, s0, R.n, queues(P, pr.kind), 0, grid, st));
     P->inflight += R.n;
     *g_out = g;
     return DSMC_OK;

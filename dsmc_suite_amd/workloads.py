@@ -73,6 +73,23 @@ def body_params(lib, body: Body, n: int, lon: float, seed: int, hd: int = 40) ->
     return p
 
 
+def coll_params(lib, body: Body, n: int, lon: float, seed: int, hd: int = 40):
+    """(phase 1, phase 2) of a physical collisional run of `body`: the ballistic
+    preset keeping 64 speed bins to 2000 m/s per cell, sublimation x10; phase 2
+    with dt 5 s, hard spheres d = 4.6e-10 m, degree-6 partner fits and the
+    second random stream."""
+    p1 = body_params(lib, body, n, lon, seed, hd)
+    p1.sublimation_scale = 10.0
+    p1.vhist_bins = 64
+    p1.vhist_max = 2000.0
+    p1.fit_degree = 6
+    p1.cross_section = math.pi * 4.6e-10 ** 2
+    p2 = p1.copy()
+    p2.dt = 5.0
+    p2.stream = 2
+    return p1, p2
+
+
 @dataclass
 class RunSpec:
     body: Body

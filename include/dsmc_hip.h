@@ -295,7 +295,7 @@ int dsmc_autotune(const dsmc_params *p, const dsmc_surface *s, dsmc_launch_shape
 /* A continuous-batching server.  Requests (physical runs of one grid size and
  * icosphere level) are prepared on the host at submission, queued in arrival
  * order and admitted into one device pool as soon as particle slots (the HBM
- * budget less max_inflight grid slots, at 268 B per particle, 332 B with
+ * budget less max_inflight grid slots, at 276 B per particle, 340 B with
  * per-particle outputs), a grid slot and table slots are free; they join the
  * particles already in flight and complete independently.  A request's
  * per-particle results do not depend on what else runs with it; its grid sums
@@ -306,12 +306,21 @@ typedef struct dsmc_server_stats {
     int64_t queued, running, particles_in_flight, slots, rounds;
     dsmc_launch_shape shape;
 } dsmc_server_stats;
-dsmc_server *dsmc_server_create(size_t hbm_budget_bytes, int max_inflight, int per_particle);
+enum dsmc_serve_flags {
+    DSMC_SERVE_PER_PARTICLE = 1,   /* keep per-particle outputs (64 B per particle slot) */
+    DSMC_SERVE_COLLISIONAL = 2     /* accept collisional requests (histogram and fit slots) */
+};
+dsmc_server *dsmc_server_create(size_t hbm_budget_bytes, int max_inflight, int flags);
 int dsmc_server_set_shape(dsmc_server *sv, const dsmc_launch_shape *shape);   /* while idle */
 /* r must stay valid until the request completes; r->elapsed_ms receives the time
  * from submission to completion. */
 int dsmc_server_submit(dsmc_server *sv, const dsmc_params *p, const dsmc_surface *s,
                        dsmc_result *r, uint64_t *ticket);
+/* A physical collisional request: phase 1 (p1, speed histograms kept), then
+ * phase 2 (p2, same grid) with the phase-1 density and fits built on the
+ * device; r receives the phase-2 results. */
+int dsmc_server_submit_collisional(dsmc_server *sv, const dsmc_params *p1, const dsmc_params *p2,
+                                   const dsmc_surface *s, dsmc_result *r, uint64_t *ticket);
 /* Admits what fits, keeps the device busy, completes finished requests; waits
  * for at most one replay of the round graph.  Returns requests completed. */
 int dsmc_server_step(dsmc_server *sv);

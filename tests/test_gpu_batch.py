@@ -189,3 +189,50 @@ def test_server_budget_and_cancel(lib):
         assert lib.dsmc_server_poll(sv, ta.value) == -1
     finally:
         lib.dsmc_server_destroy(sv)
+
+
+def test_server_collisional_alone_mix_and_pipeline(lib):
+    """A collisional request served alone, served among ballistic requests, and
+    run through dsmc_run_collisional: the same phase-2 particles bit for bit."""
+    surf = W.icosphere(lib, LEVELS, W.BODIES[0], 0.0)
+    p1, p2 = W.coll_params(lib, W.BODIES[0], 1500, 0.0, 71, hd=HD)
+    p1.sublimation_scale = p2.sublimation_scale = 1e-6      # a moderately collisional exosphere
+    p2.nparticles = 600
+    ref = Buffers(p2, 5, per_particle=True, moments=True)
+    assert lib.dsmc_run_collisional(C.byref(p1), C.byref(p2), C.byref(surf.surface), None, C.byref(ref.result)) == 0
+    cref = {k: int(getattr(ref.result, k)) for k in COUNTERS}
+    assert cref["n_collisions"] > 10, cref
+    sp = specs()
+    params, surfs = W.build_runs(lib, sp, levels=LEVELS, hd=HD)
+    sv = lib.dsmc_server_create(4 << 30, 8, 3)          # per-particle outputs, collisional requests
+    assert sv
+    try:
+        ra = Buffers(p2, 5, per_particle=True, moments=True)
+        t = C.c_uint64()
+        assert lib.dsmc_server_submit_collisional(sv, C.byref(p1), C.byref(p2), C.byref(surf.surface),
+                                                  C.byref(ra.result), C.byref(t)) == 0
+        assert lib.dsmc_server_wait(sv, t.value) == 0
+        assert_same_run(ref, cref, ra, {k: int(getattr(ra.result, k)) for k in COUNTERS}, "served alone")
+        mix = Buffers(p2, 5, per_particle=True, moments=True)
+        others = [Buffers(p, 5, per_particle=True, moments=True) for p in params[:3]]
+        tk = []
+        for j in range(3):
+            tt = C.c_uint64()
+            assert lib.dsmc_server_submit(sv, C.byref(params[j]), C.byref(surfs[j].surface),
+                                          C.byref(others[j].result), C.byref(tt)) == 0
+            tk.append(tt.value)
+            if j == 1:
+                assert lib.dsmc_server_submit_collisional(sv, C.byref(p1), C.byref(p2), C.byref(surf.surface),
+                                                          C.byref(mix.result), C.byref(t)) == 0
+                tk.append(t.value)
+            assert lib.dsmc_server_step(sv) >= 0
+        for x in tk:
+            assert lib.dsmc_server_wait(sv, x) == 0
+        assert_same_run(ref, cref, mix, {k: int(getattr(mix.result, k)) for k in COUNTERS}, "served in a mix")
+        for j in range(3):
+            b1, c1 = alone(lib, params[j], surfs[j])
+            assert_same_run(b1, c1, others[j], {k: int(getattr(others[j].result, k)) for k in COUNTERS},
+                            f"ballistic request {j} beside a collisional one")
+    finally:
+        lib.dsmc_server_destroy(sv)
+
