@@ -114,6 +114,11 @@ def plain_run(lib, surf, n, args, shape) -> dict:
                          "flight_kernel<1,0>", "kernel_avg_ms_per_run": favg * 1e3}}
 
 
+def progress(msg: str) -> None:
+    """One progress line on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def percentiles(x) -> dict:
     x = np.asarray(x, dtype=np.float64)
     if x.size == 0:
@@ -182,13 +187,14 @@ def batched_run(lib, args, shape) -> dict:
 def collisional_run(lib, args) -> dict:
     """The collisional workload: Ceres H2O, phase 1 (collisionless, speed
     histograms) then phase 2 (hard spheres d = 4.6e-10 m, dt 5 s, degree-6
-    partner fits), sublimation x10, grid 50x36x72, both phases at
+    partner fits), emission 2e26 molecules/s x10, grid 50x36x72, both phases at
     args.coll_particles, through the device-resident two-phase pipeline."""
     surf = icosphere(lib, 3)
     n = args.coll_particles
-    p1 = capi.preset(lib, capi.PRESET_CERES_PHYSICAL, 36, 72, n=n, seed=args.seed)
-    p1.sublimation_scale = 10.0
     p2 = capi.preset(lib, capi.PRESET_CERES_PHYSICAL_COLL, 36, 72, n=n, seed=args.seed)
+    p1 = capi.preset(lib, capi.PRESET_CERES_PHYSICAL, 36, 72, n=n, seed=args.seed)
+    p1.production_rate = p2.production_rate      # ~6 kg/s of water ...
+    p1.sublimation_scale = p2.sublimation_scale  # ... x10
     r1 = Buffers(p1, 5, per_particle=False, moments=True, vhist=True)
     r2 = Buffers(p2, 5, per_particle=False, moments=True)
     t0 = time.perf_counter()
@@ -227,7 +233,11 @@ def served_run(lib, args, shape) -> dict:
     t0 = time.perf_counter()
     i, outstanding = 0, set()
     first = True
+    t_report = 0.0
     while i < len(stream) or outstanding:
+        if time.perf_counter() - t0 > t_report + 5.0:
+            t_report = time.perf_counter() - t0
+            progress(f"served: {i}/{len(stream)} submitted, {len(outstanding)} outstanding at {t_report:.0f} s")
         now = time.perf_counter() - t0
         while i < len(stream) and stream[i][0] <= now:
             tk = C.c_uint64()
@@ -265,8 +275,11 @@ def served_run(lib, args, shape) -> dict:
     lat = [bufs[j].result.elapsed_ms + (t_sub[j] - stream[j][0]) * 1e3 for j in range(len(stream))]
     steps = int(stats.particle_steps)
     ncoll = sum(1 for s in specs if s.collisional)
+    # offered load: the stream's whole work over the arrival window; above the
+    # served rate the queue grows for the whole window and latency with it
     return {"value": steps / t_end, "unit": "particle-steps/s", "requests": len(stream),
             "collisional_requests": ncoll, "offered_seconds": args.serve_seconds,
+            "offered_steps_per_s": steps / args.serve_seconds,
             "rate_per_s": args.serve_rate, "wall_s": t_end, "particles": int(sum(s.n for s in specs)),
             "particle_steps": steps, "latency": percentiles(lat), "rounds": int(stats.rounds),
             "slots": int(stats.slots)}
@@ -294,7 +307,7 @@ def main():
     ap.add_argument("--sequential", type=int, default=1, help="also time the batch's runs one by one")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=20.0)
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-scale", type=float, default=1.0)
     ap.add_argument("--serve-budget-gb", type=int, default=128)
     ap.add_argument("--serve-collisional", type=float, default=0.3, help="fraction of collisional requests")
@@ -312,7 +325,9 @@ def main():
     if args.block or args.blocks_per_cu or args.relaunch_batch or args.step_budget or args.service_inline:
         shape = capi.LaunchShape(args.block or 256, args.blocks_per_cu or 4, args.relaunch_batch or 16,
                                  args.step_budget or 1024, args.service_inline)
+    progress(f"plain run at {args.particles:.0e} particles")
     main_run = plain_run(lib, surf, args.particles, args, shape)
+    progress(f"plain: {main_run['value']:.3e} particle-steps/s")
     p = main_run.pop("params")
     line = {
         "metric": METRIC, "value": main_run["value"], "unit": "particle-steps/s", "n_gpus": 1,
@@ -326,6 +341,7 @@ def main():
         "roofline": main_run["roofline"],
     }
     if args.also_particles:
+        progress(f"plain run at {args.also_particles:.0e} particles")
         other = plain_run(lib, surf, args.also_particles, args, shape)
         other.pop("params")
         line["config"][f"plain_{args.also_particles:.0e}"] = {
@@ -333,17 +349,24 @@ def main():
             "flight_kernel_ms": other["flight_kernel_ms"], "rounds": other["rounds"]}
     tuned = None
     if not args.no_batch:
+        progress("batched: 64 runs (autotune, launches, plain sequential)")
         bt = batched_run(lib, args, shape)
+        progress(f"batched: {bt['value']:.3e} particle-steps/s")
         line["batched"] = bt
         s = bt["shape"]
         tuned = capi.LaunchShape(s["block"], s["blocks_per_cu"], s["relaunch_batch"], s["step_budget"],
                                  s["service_inline"])
         line["batched_vs_plain"] = bt["value"] / main_run["value"]
     if not args.no_coll:
+        progress(f"collisional pipeline at {args.coll_particles:.0e} particles per phase")
         line["collisional"] = collisional_run(lib, args)
+        progress(f"collisional: {line['collisional']['value']:.3e} particle-steps/s")
     if not args.no_serve:
+        progress(f"served: {args.serve_rate} req/s for {args.serve_seconds} s")
         line["served"] = served_run(lib, args, shape if shape is not None else tuned)
+        progress(f"served: {line['served']['value']:.3e} particle-steps/s")
     if not args.no_cpu:
+        progress("CPU baseline")
         threads = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(p, surf, args.cpu_sample, threads)
     print(json.dumps(line))

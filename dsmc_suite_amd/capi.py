@@ -40,7 +40,8 @@ class Params(C.Structure):
         ("record_cap_cell", C.c_int), ("record_cap_total", C.c_int64),
         ("cross_section", C.c_double), ("fit_degree", C.c_int), ("max_per_cell", C.c_int),
         ("min_fit_samples", C.c_int), ("density_scale", C.c_double),
-        ("sublimation_scale", C.c_double), ("vhist_bins", C.c_int), ("vhist_max", C.c_double),
+        ("sublimation_scale", C.c_double), ("production_rate", C.c_double),
+        ("vhist_bins", C.c_int), ("vhist_max", C.c_double),
     ]
 
     def copy(self) -> "Params":

@@ -465,7 +465,7 @@ namespace dsmc {
 void number_density(const dsmc_params *p1, double wsum, const int64_t *vcount, double *ndens)
 {
     const double R = p1->radius;
-    const double F = wsum * R * R * p1->sublimation_scale;   /* molecules per second */
+    const double F = (p1->production_rate > 0 ? p1->production_rate : wsum * R * R) * p1->sublimation_scale;
     const int ftd = p1->ftd, ffd = p1->ffd, hd = p1->hd;
     std::vector<double> redge(hd + 1), sl(ftd + 1);
     for (int k = 0; k <= hd; k++)

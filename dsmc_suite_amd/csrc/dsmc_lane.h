@@ -711,10 +711,34 @@ DSMC_HD int locate_face_near(const DsmcEnv &E, int hint, double dx, double dy, d
     return locate_face(E, dx, dy, dz);
 }
 
-DSMC_HD double poly_eval(const double *c, int deg, double x)
+/* A fit polynomial held in registers for the root searches below (degree at
+ * most 6, checked where runs are staged): the coefficients, highest power
+ * first, right-aligned in DSMC_FIT_MAX_DEG + 1 slots behind leading zeros.
+ * Horner's rule over the zeros keeps v = +0 until the first real coefficient
+ * (0 * x + 0 = +0 for every finite x; a NaN or infinite x gives NaN either way),
+ * so fit_eval returns exactly what Horner's rule over the deg + 1 coefficients
+ * alone gives. */
+#define DSMC_FIT_MAX_DEG 6
+struct FitPoly {
+    double c[DSMC_FIT_MAX_DEG + 1];
+};
+
+DSMC_HD FitPoly fit_load(const double *c, int deg)
+{
+    FitPoly f;
+#pragma unroll
+    for (int i = 0; i <= DSMC_FIT_MAX_DEG; i++) {
+        const int j = i - (DSMC_FIT_MAX_DEG - deg);
+        f.c[i] = j >= 0 ? c[j] : 0.0;
+    }
+    return f;
+}
+
+DSMC_HD double fit_eval(const FitPoly &f, double x)
 {
     double v = 0.0;
-    for (int i = 0; i <= deg; i++) v = v * x + c[i];
+#pragma unroll
+    for (int i = 0; i <= DSMC_FIT_MAX_DEG; i++) v = v * x + f.c[i];
     return v;
 }
 
@@ -722,16 +746,17 @@ DSMC_HD double poly_eval(const double *c, int deg, double x)
  * histogram in normalised speed x = (v - min_v) / (max_v - min_v), P(0) = 50;
  * the speed with P(x) = u (50 - P(1)) + P(1), by bisection (the sign at the
  * left end tracked, an exact zero ends it) to a 0.01 m/s bracket. */
-DSMC_HD double sample_speed_phys(const double *c, int deg, double min_v, double max_v, double u)
+DSMC_HD double sample_speed_phys(const double *cg, int deg, double min_v, double max_v, double u)
 {
-    const double pv = poly_eval(c, deg, 1.0);
+    const FitPoly c = fit_load(cg, deg);
+    const double pv = fit_eval(c, 1.0);
     const double span = max_v - min_v;
     const double ranking = u * ((double)DSMC_VIRTUAL_RECORDS - pv) + pv;
     double x0 = 0.0, x1 = 1.0;
-    double f0 = poly_eval(c, deg, 0.0) - ranking;
+    double f0 = fit_eval(c, 0.0) - ranking;
     for (int iter = 0; iter < 100; iter++) {
         double xm = (x0 + x1) * 0.5;
-        double fm = poly_eval(c, deg, xm) - ranking;
+        double fm = fit_eval(c, xm) - ranking;
         if (fm == 0.0) { x0 = x1 = xm; break; }
         if ((fm < 0.0) == (f0 < 0.0)) { x0 = xm; f0 = fm; }
         else x1 = xm;
@@ -887,15 +912,16 @@ DSMC_HD void lane_service(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
 /* ------------------------------------------------------------------ collision (advanced) */
 
 /* collision.c:43-101 */
-DSMC_HD double sample_speed(const double *c, int deg, double min_v, double max_v, double u, int nn)
+DSMC_HD double sample_speed(const double *cg, int deg, double min_v, double max_v, double u, int nn)
 {
-    double val_at_max = poly_eval(c, deg, max_v);
+    const FitPoly c = fit_load(cg, deg);
+    double val_at_max = fit_eval(c, max_v);
     double ranking = u * ((double)nn - val_at_max) + val_at_max;
     double v0 = min_v, v1 = max_v;
     for (int iter = 0; iter < 100; iter++) {
         double vm = (v0 + v1) * 0.5;
-        double fm = poly_eval(c, deg, vm) - ranking;
-        double f1 = poly_eval(c, deg, v1) - ranking;
+        double fm = fit_eval(c, vm) - ranking;
+        double f1 = fit_eval(c, v1) - ranking;
         if (fm * f1 < 0) v0 = vm;
         else v1 = vm;
         if (fabs(v1 - v0) < 0.01) break;

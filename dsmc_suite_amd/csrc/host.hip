@@ -442,6 +442,8 @@ extern "C" int dsmc_params_preset(dsmc_params *p, int preset, int ftd, int ffd)
         p->vhist_bins = 64;                  /* phase-1 speed histograms: 31.25 m/s bins */
         p->vhist_max = 2000.0;
         if (preset == DSMC_PRESET_CERES_PHYSICAL_COLL) {
+            /* ~6 kg/s of water: the order of Ceres' measured outgassing, x10 */
+            p->production_rate = 2.0e26;
             p->sublimation_scale = 10.0;
             p->stream = DSMC_STREAM_ADVANCED;   /* phase 2 draws apart from phase 1 */
         }

@@ -351,6 +351,41 @@ __device__ __attribute__((noinline)) void phys_service(DsmcLane &T, const DsmcEn
     lane_service<Ops, 1, ADV>(T, E, cnt);
 }
 
+/* A collision (physical collisional runs), out of line with only what it reads
+ * and writes passed in registers: velocity, random-stream counter, cell. */
+struct CollideOut {
+    double vx, vy, vz, col_p;
+    uint64_t k;
+    int num_col;
+};
+
+__device__ __attribute__((noinline)) CollideOut phys_collide(double vx, double vy, double vz, double col_p, uint64_t k,
+                                                             int64_t pid, int ht, int td, int fd, const DsmcEnv *Ep)
+{
+    const DsmcEnv E = *Ep;
+    DsmcLane T;
+    T.vx = vx; T.vy = vy; T.vz = vz; T.col_p = col_p;
+    T.k = k; T.pid = pid; T.blk = ~0ull;
+    T.ht = ht; T.td = td; T.fd = fd;
+    T.num_col = 0;
+    lane_collide(T, E);
+    CollideOut o = {T.vx, T.vy, T.vz, T.col_p, T.k, T.num_col};
+    return o;
+}
+
+#define COLLIDE_BATCH 16
+
+__device__ __forceinline__ void lane_collide_lean(DsmcLane &L, const DsmcEnv *Ep)
+{
+    CollideOut o = phys_collide(L.vx, L.vy, L.vz, L.col_p, L.k, L.pid, L.ht, L.td, L.fd, Ep);
+    L.vx = o.vx; L.vy = o.vy; L.vz = o.vz;
+    L.k = o.k; L.blk = ~0ull;
+    L.col_p = o.col_p;
+    L.real_col_p = 0.0;
+    L.num_col += o.num_col;
+    L.phase = LANE_FLIGHT;
+}
+
 /* Physical particles: flight with the landing service inline.  A lane whose hop
  * ends lands (freeze-out or re-emission) or finishes its particle in the same
  * launch, so a particle makes as many hops per launch as its step budget
@@ -425,11 +460,18 @@ phys_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcPoolDev P, DsmcRec *rec, DsmcQueu
                 else lane_step<DsmcOpsLds, 1, ADV>(L, E0, TB);
                 if (L.phase == LANE_FLIGHT && L.steps >= budget_end) L.phase = LANE_IDLE + 8;
             }
+            /* collisions in a dense gas come every few steps: once COLLIDE_BATCH
+             * lanes wait for one they take the lean path here, without ending
+             * the flight loop; fewer wait for the service below */
+            if (ADV && __popcll(__ballot(L.phase == LANE_COLLIDE)) >= COLLIDE_BATCH && L.phase == LANE_COLLIDE)
+                lane_collide_lean(L, POOL ? P.envs + run : Ed);
             uint64_t ended = __ballot(L.phase != LANE_FLIGHT && L.phase != LANE_IDLE);
             uint64_t waiting = ended | __ballot(L.phase == LANE_IDLE);
             if ((ended && __popcll(waiting) >= svc_batch) || waiting == __ballot(true)) break;
         }
-        /* landings, collisions and ended particles: service inline */
+        /* the remaining collisions: the lean path (the deposit visit stays open) */
+        if (ADV && __ballot(L.phase == LANE_COLLIDE) && L.phase == LANE_COLLIDE) lane_collide_lean(L, POOL ? P.envs + run : Ed);
+        /* landings and ended particles: service inline */
         DsmcCounts cnt = {0, 0, 0, 0, 0, 0, 0, 0};
         const bool svc = L.phase == LANE_LANDED || L.phase == LANE_ENDED || L.phase == LANE_COLLIDE;
         if (svc) {

@@ -18,6 +18,7 @@ import numpy as np
 from .capi import PRESET_CERES_PHYSICAL, Params, SurfaceArrays, preset, ptr
 
 K_B = 1.380649e-23
+CERES_PRODUCTION = 2.0e26     # molecules/s, the order of Ceres' measured water outgassing
 
 
 @dataclass(frozen=True)
@@ -75,10 +76,12 @@ def body_params(lib, body: Body, n: int, lon: float, seed: int, hd: int = 40) ->
 
 def coll_params(lib, body: Body, n: int, lon: float, seed: int, hd: int = 40):
     """(phase 1, phase 2) of a physical collisional run of `body`: the ballistic
-    preset keeping 64 speed bins to 2000 m/s per cell, sublimation x10; phase 2
-    with dt 5 s, hard spheres d = 4.6e-10 m, degree-6 partner fits and the
-    second random stream."""
+    preset keeping 64 speed bins to 2000 m/s per cell; emission Ceres' outgassing
+    per unit area (2e26 molecules/s over R = 470 km) x10; phase 2 with dt 5 s,
+    hard spheres d = 4.6e-10 m, degree-6 partner fits and the second random
+    stream."""
     p1 = body_params(lib, body, n, lon, seed, hd)
+    p1.production_rate = CERES_PRODUCTION * (body.radius / 470e3) ** 2
     p1.sublimation_scale = 10.0
     p1.vhist_bins = 64
     p1.vhist_max = 2000.0

@@ -113,7 +113,9 @@ typedef struct dsmc_params {
     double density_scale;     /* multiplies the phase-1 density read as copy_atm */
 
     /* physical-mode emission */
-    double sublimation_scale; /* multiplies face sublimation weights (collisional runs: x10) */
+    double sublimation_scale; /* multiplies the emission rate (collisional runs: x10) */
+    double production_rate;   /* collisional runs: emission rate [molecules/s] behind the
+                                 phase-1 number density; 0 = sum of face weights x R^2 */
 
     /* physical collisional runs: phase-1 speed histograms per cell (one sample
      * per cell visit: the speed of the visit's mean velocity) */

@@ -1155,15 +1155,15 @@ void cpu_ref_hist_fits(const dsmc_params *p, const int64_t *vhist, double *coeff
     }
 }
 
-/* Number density of a phase-1 run: vcount x dt x (sum of face weights x R^2 x
- * sublimation_scale) / (nparticles x cell volume). */
+/* Number density of a phase-1 run: vcount x dt x F / (nparticles x cell volume),
+ * F = (production_rate, or the sum of face weights x R^2) x sublimation_scale. */
 int cpu_ref_number_density(const dsmc_params *p1, const dsmc_surface *s, const int64_t *vcount, double *ndens)
 {
     const int nfine = 20 << (2 * s->levels);
     double F = 0.0;
     for (int f = 0; f < nfine; f++) F += s->face_w[f];
     const double R = p1->radius;
-    F = F * R * R * p1->sublimation_scale;
+    F = (p1->production_rate > 0 ? p1->production_rate : F * R * R) * p1->sublimation_scale;
     const int ftd = p1->ftd, ffd = p1->ffd, hd = p1->hd;
     double *redge = malloc(sizeof(double) * (hd + 1)), *sl = malloc(sizeof(double) * (ftd + 1));
     for (int k = 0; k <= hd; k++)

@@ -141,6 +141,7 @@ def test_number_density_closed_form(lib):
     o = orc(False)
     surf = surface(lib)
     p = coll_params(lib, 1000, 1, hd=20)
+    p.production_rate = 0.0                     # the emission rate of the face weights
     rng = np.random.default_rng(5)
     cells = p.ftd * p.ffd * p.hd
     vcount = rng.integers(0, 10_000, cells).astype(np.int64)
@@ -154,6 +155,10 @@ def test_number_density_closed_form(lib):
         * (2 * np.pi / p.ffd) * np.ones((1, 1, p.ffd))
     want = vcount.reshape(p.hd, p.ftd, p.ffd) * p.dt * F / (p.nparticles * V)
     assert np.allclose(nd, want.ravel(), rtol=1e-12, atol=0)
+    q = p.copy()
+    q.production_rate = 3.5e26                  # an explicit emission rate replaces the face sum
+    nd2 = density(o, q, surf, vcount)
+    assert np.allclose(nd2, nd * (3.5e26 * q.sublimation_scale) / F, rtol=1e-12, atol=0)
     got = np.zeros(cells)
     assert lib.dsmc_number_density(C.byref(p), C.byref(surf.surface), ptr(vcount, C.c_int64),
                                    ptr(got, C.c_double)) == 0

@@ -196,7 +196,6 @@ def test_server_collisional_alone_mix_and_pipeline(lib):
     run through dsmc_run_collisional: the same phase-2 particles bit for bit."""
     surf = W.icosphere(lib, LEVELS, W.BODIES[0], 0.0)
     p1, p2 = W.coll_params(lib, W.BODIES[0], 1500, 0.0, 71, hd=HD)
-    p1.sublimation_scale = p2.sublimation_scale = 1e-6      # a moderately collisional exosphere
     p2.nparticles = 600
     ref = Buffers(p2, 5, per_particle=True, moments=True)
     assert lib.dsmc_run_collisional(C.byref(p1), C.byref(p2), C.byref(surf.surface), None, C.byref(ref.result)) == 0
