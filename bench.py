@@ -6,7 +6,7 @@ Workload (default): the Ceres H2O ballistic exosphere, physical mode - icosphere
 by Antoine sublimation flux, flux-Maxwellian emission, FP64 Stormer-Verlet with
 dt = 10 s and at most 20000 steps, photolysis lifetime 6.4e5 s, freeze-out below
 110 K, grid 50 exponential shells R..20R x 36 x 72 with density and per-cell
-velocity moments - at 1e6 particles (synthetic inputs, seeded).
+velocity moments - at 1e7 particles (synthetic inputs, seeded); also at 1e6.
 
 A "step" of this script is one complete run of that workload with its inputs
 already resident in device memory (dsmc_run_create stages them once).  The line
@@ -38,6 +38,12 @@ METRIC = "particle-steps/sec FP64, plain vs batched/served, 1 MI355X; p50/p99 ru
 # cell/photolysis/deposit bookkeeping 15), counted as in the design notes.
 FLOPS_PER_STEP = 66
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per launch of the flight kernel on the 1e7-particle Ceres run with
+# the default launch shape, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# (per dispatch, averaged over the run's launches), each scaled by what the
+# counter reads for a plain 1 GiB device-to-device copy (FETCH_SIZE x 2.00,
+# WRITE_SIZE x 1.00).  Only that configuration reports it; others give null.
+PMC_TRAFFIC_1E7 = 310_172_860 + 7_782_874_893      # read + written, bytes per launch
 
 
 def icosphere(lib, levels, lon=0.0, mass=2.99e-26, kB=1.380649e-23):
@@ -100,18 +106,26 @@ def plain_run(lib, surf, n, args, shape) -> dict:
     lib.dsmc_run_destroy(run)
     steps_per_run = outs.result.total_steps
     favg = float(np.mean(fms)) / 1e3          # flight-kernel device time per run
+    launches = max(rounds.value, 1)           # flight launches per run
     achieved = FLOPS_PER_STEP * steps_per_run / favg / 1e12
+    traffic = PMC_TRAFFIC_1E7 if (shape is None and n == 10_000_000 and not args.no_moments) else None
     return {"params": p, "value": steps_per_run * args.steps / wall, "ms_per_run": wall / args.steps * 1e3,
             "particles": n, "particle_steps_per_run": steps_per_run,
             "fates": {k: int(getattr(outs.result, k)) for k in
                       ("n_escaped", "n_frozen", "n_photolysed", "n_capped")},
             "run_device_ms": [round(x, 3) for x in kms],
             "flight_kernel_ms": [round(x, 3) for x in fms], "rounds": rounds.value,
-            "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                         "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            # compute-bound FP64: the push runs on the vector ALUs (no matrix
+            # cores); MI355X's FP64 vector and FP64 matrix peaks are both 78.6
+            "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                         "compute": "FP64 vector ALU (VALU), not matrix cores",
                          "flops_per_particle_step": FLOPS_PER_STEP,
+                         "flops_per_launch": FLOPS_PER_STEP * steps_per_run / launches,
+                         "launches_per_run": launches,
                          "kernel": "phys_kernel<false>" if (shape is not None and shape.service_inline) else
-                         "flight_kernel<1,0>", "kernel_avg_ms_per_run": favg * 1e3}}
+                         "flight_kernel<1,0>", "kernel_avg_ms_per_launch": favg * 1e3 / launches,
+                         "kernel_ms_per_run": favg * 1e3}}
 
 
 def progress(msg: str) -> None:
@@ -285,6 +299,20 @@ def served_run(lib, args, shape) -> dict:
             "slots": int(stats.slots)}
 
 
+def copy_calibration(mib: int) -> None:
+    """A plain device-to-device copy of known size: what the HBM byte counters
+    read for it (per dispatch) against its 2 x size algorithmic bytes gives the
+    factor that scales the counters of the particle kernels."""
+    import torch
+    n = mib << 20
+    a = torch.ones(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "copies": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,11 +341,16 @@ def main():
     ap.add_argument("--serve-collisional", type=float, default=0.3, help="fraction of collisional requests")
     ap.add_argument("--no-coll", action="store_true")
     ap.add_argument("--coll-particles", type=int, default=2_000_000)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only copy a buffer of this many MiB device to device 5 times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+                                   "(the path is not sharded across GPUs)"}))
         sys.exit(2)
+    if args.calibrate_copy_mib:
+        copy_calibration(args.calibrate_copy_mib)
+        return
 
     lib = capi.load()
     surf = icosphere(lib, 3)

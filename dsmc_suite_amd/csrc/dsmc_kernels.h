@@ -35,6 +35,7 @@ hipError_t dsmc_launch_start(const DsmcEnv *E, int32_t run_id, DsmcRec *rec, int
 hipError_t dsmc_launch_flight(const DsmcEnv *E, DsmcRec *rec, DsmcQueues Q, int p, int grid, int block,
                               int refill_batch, int64_t step_budget, hipStream_t st);
 /* service kernel of round p (one run), then the queue reset */
+extern "C" int dsmc_flight_occupancy(const DsmcEnv *E, int inline_svc, int block);
 hipError_t dsmc_launch_service(const DsmcEnv *E, DsmcRec *rec, DsmcQueues Q, int p, int grid, hipStream_t st);
 /* the same two for a pool of runs */
 hipError_t dsmc_launch_pool_flight(const DsmcPoolDev *P, DsmcQueues Q, int p, int grid, int block,

@@ -627,6 +627,25 @@ extern "C" hipError_t dsmc_launch_flight(const DsmcEnv *E, DsmcRec *rec, DsmcQue
     return hipGetLastError();
 }
 
+/* Blocks of `block` threads of the flight kernel (inline_svc: the kernel with
+ * the landing service inline) that one CU holds at once, 0 on error. */
+extern "C" int dsmc_flight_occupancy(const DsmcEnv *E, int inline_svc, int block)
+{
+    if (!block_ok(block)) return 0;
+    int nb = 0;
+    hipError_t e = hipErrorInvalidValue;
+    const size_t lds = sizeof(double) * (size_t)((E->ftd + 1) + (E->hd + 2) + 2 * (E->ffd + 1));
+    const int mode = E->mode, adv = E->advanced;
+    if (inline_svc && mode == 1) {
+        if (adv) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, phys_kernel<false, 1>, block, lds);
+        else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, phys_kernel<false, 0>, block, lds);
+    } else if (mode == 0 && !adv) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flight_kernel<0, 0>, block, lds);
+    else if (mode == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flight_kernel<0, 1>, block, lds);
+    else if (!adv) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flight_kernel<1, 0>, block, lds);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flight_kernel<1, 1>, block, lds);
+    return e == hipSuccess ? nb : 0;
+}
+
 extern "C" hipError_t dsmc_launch_service(const DsmcEnv *E, DsmcRec *rec, DsmcQueues Q, int p, int grid,
                                           hipStream_t st)
 {
