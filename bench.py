@@ -225,12 +225,12 @@ def collisional_run(lib, args) -> dict:
                              ("n_escaped", "n_frozen", "n_photolysed", "n_capped")}}
 
 
-def served_run(lib, args, shape) -> dict:
-    """Poisson request stream (20 req/s) of runs, counts log-uniform on [1e4,
-    4e6], body and subsolar longitude drawn per request, 30% collisional (phase 1
-    + phase 2 at the request's count), through the continuous-batching server;
-    latency = arrival to result ready."""
-    stream = W.request_stream(args.serve_rate, args.serve_seconds, seed=5, scale=args.serve_scale,
+def served_run(lib, args, shape, rate: float, seconds: float) -> dict:
+    """Poisson request stream (`rate` req/s for `seconds`) of runs, counts
+    log-uniform on [1e4, 4e6], body and subsolar longitude drawn per request, 30%
+    collisional (phase 1 + phase 2 at the request's count), through the
+    continuous-batching server; latency = arrival to result ready."""
+    stream = W.request_stream(rate, seconds, seed=5, scale=args.serve_scale,
                               frac_collisional=args.serve_collisional)
     specs = [s for _, s in stream]
     params, surfs = W.build_runs(lib, specs, levels=4, hd=40)
@@ -292,9 +292,9 @@ def served_run(lib, args, shape) -> dict:
     # offered load: the stream's whole work over the arrival window; above the
     # served rate the queue grows for the whole window and latency with it
     return {"value": steps / t_end, "unit": "particle-steps/s", "requests": len(stream),
-            "collisional_requests": ncoll, "offered_seconds": args.serve_seconds,
-            "offered_steps_per_s": steps / args.serve_seconds,
-            "rate_per_s": args.serve_rate, "wall_s": t_end, "particles": int(sum(s.n for s in specs)),
+            "collisional_requests": ncoll, "offered_seconds": seconds,
+            "offered_steps_per_s": steps / seconds,
+            "rate_per_s": rate, "wall_s": t_end, "particles": int(sum(s.n for s in specs)),
             "particle_steps": steps, "latency": percentiles(lat), "rounds": int(stats.rounds),
             "slots": int(stats.slots)}
 
@@ -336,6 +336,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-sustained-rate", type=float, default=2.0, help="second served leg (0: none)")
+    ap.add_argument("--serve-sustained-seconds", type=float, default=10.0)
     ap.add_argument("--serve-scale", type=float, default=1.0)
     ap.add_argument("--serve-budget-gb", type=int, default=128)
     ap.add_argument("--serve-collisional", type=float, default=0.3, help="fraction of collisional requests")
@@ -395,9 +397,18 @@ def main():
         line["collisional"] = collisional_run(lib, args)
         progress(f"collisional: {line['collisional']['value']:.3e} particle-steps/s")
     if not args.no_serve:
+        sshape = shape if shape is not None else tuned
         progress(f"served: {args.serve_rate} req/s for {args.serve_seconds} s")
-        line["served"] = served_run(lib, args, shape if shape is not None else tuned)
+        line["served"] = served_run(lib, args, sshape, args.serve_rate, args.serve_seconds)
         progress(f"served: {line['served']['value']:.3e} particle-steps/s")
+        if args.serve_sustained_rate > 0:
+            # the same stream at a rate the device keeps up with: latency is
+            # then service time, not time spent queued behind a backlog
+            progress(f"served (sustainable load): {args.serve_sustained_rate} req/s for "
+                     f"{args.serve_sustained_seconds} s")
+            line["served_sustained"] = served_run(lib, args, sshape, args.serve_sustained_rate,
+                                                  args.serve_sustained_seconds)
+            progress(f"served (sustainable load): p50 {line['served_sustained']['latency']['p50_ms']:.0f} ms")
     if not args.no_cpu:
         progress("CPU baseline")
         threads = min(16, os.cpu_count() or 1)

@@ -240,9 +240,10 @@ int dsmc_elementary(int fn, const double *x, const double *y, double *out, int n
 /* Physical collisional runs.
  *
  * Number density of a phase-1 run: the steps spent in each cell (vcount) x dt x
- * the body's emission rate (sum of face weights x R^2 x sublimation_scale,
- * molecules/s) / (nparticles x cell volume); cells [ht][td][fd], spherical-shell
- * sectors between the grid's radial, latitude and longitude edges. */
+ * the body's emission rate (production_rate, or the sum of face weights x R^2
+ * when it is 0; times sublimation_scale; molecules/s) / (nparticles x cell
+ * volume); cells [ht][td][fd], spherical-shell sectors between the grid's
+ * radial, latitude and longitude edges. */
 int dsmc_number_density(const dsmc_params *p1, const dsmc_surface *s, const int64_t *vcount, double *ndens);
 /* Partner-speed fits from phase-1 histograms (device): per cell with more than
  * min_fit_samples steps, the reverse cumulative histogram P = 50 (steps at or
