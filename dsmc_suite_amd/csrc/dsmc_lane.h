@@ -306,6 +306,48 @@ DSMC_HD int table_walk(const double *tab, int k, int kmax, double v)
     return k;
 }
 
+/* The current cell's table entries of a physical step, read before the push so
+ * that the table reads are in flight while it computes: latitude thresholds
+ * tq[td], tq[td+1], sector directions cs[2fd..2fd+3], radial thresholds
+ * hr[ht], hr[ht+1] (ht <= hd; the radial table has hd + 2 entries). */
+struct CellPre {
+    double tlo, thi, c0, s0, c1, s1, rlo, rhi;
+};
+
+DSMC_HD CellPre cell_pre(int td, int fd, int ht, const double *tq, const double *cs, const double *hr)
+{
+    CellPre P;
+    P.tlo = tq[td]; P.thi = tq[td + 1];
+    P.c0 = cs[2 * fd]; P.s0 = cs[2 * fd + 1]; P.c1 = cs[2 * fd + 2]; P.s1 = cs[2 * fd + 3];
+    P.rlo = hr[ht]; P.rhi = hr[ht + 1];
+    return P;
+}
+
+/* td_update, fd_update and table_walk with their first test on preloaded
+ * entries: each returns the unchanged index exactly when the original's first
+ * test (or both loop conditions) would, else the original runs. */
+DSMC_HD int td_update_pre(double z, double Ri, int td, int ftd, const double *tq, const CellPre &P)
+{
+    double eps = 1e-12 * Ri;
+    if (z - P.tlo * Ri > eps && P.thi * Ri - z > eps) return td;
+    return td_update(z, Ri, td, ftd, tq);
+}
+
+DSMC_HD int fd_update_pre(double x, double y, int fd, int ffd, const double *cs, const CellPre &P)
+{
+    double eps = DSMC_FD_MARGIN * (fabs(x) + fabs(y));
+    if (x * P.c0 - y * P.s0 > eps && x * P.c1 - y * P.s1 < -eps) return fd;
+    return fd_update(x, y, fd, ffd, cs);
+}
+
+/* table_walk keeps k iff (k == 0 or !(v < tab[k])) and (k == kmax or
+ * !(v >= tab[k+1])); a NaN v keeps k both ways. */
+DSMC_HD int table_walk_pre(const double *tab, int k, int kmax, double v, const CellPre &P)
+{
+    if ((k == 0 || v >= P.rlo) && (k == kmax || v < P.rhi)) return k;
+    return table_walk(tab, k, kmax, v);
+}
+
 /* ------------------------------------------------------------------ deposit */
 template <class Ops, class EnvT>
 DSMC_HD void lane_flush(DsmcLane &L, const EnvT &E)
@@ -940,6 +982,8 @@ template <class Ops, int MODE, int ADV, class EnvT>
 DSMC_HD void lane_step(DsmcLane &L, const EnvT &E, const DsmcCellTabs &TB)
 {
     const double h = E.h;
+    CellPre CP;
+    if constexpr (MODE != 0) CP = cell_pre(L.td, L.fd, L.ht, TB.tq, TB.cs, TB.hr);
     if (MODE == 0) { L.flight_time += h; L.time += h; }
     L.steps++;
     double Ri = L.Ri;
@@ -969,9 +1013,9 @@ DSMC_HD void lane_step(DsmcLane &L, const EnvT &E, const DsmcCellTabs &TB)
             L.n_day++;
             if (L.n_day >= L.n_photo) { lane_flush<Ops>(L, E); L.fate = 3; L.phase = LANE_ENDED; return; }
         }
-        L.td = td_update(nz, Rn, L.td, E.ftd, TB.tq);
-        L.fd = fd_update(nx, ny, L.fd, E.ffd, TB.cs);
-        L.ht = table_walk(TB.hr, L.ht, E.hd, Rn);
+        L.td = td_update_pre(nz, Rn, L.td, E.ftd, TB.tq, CP);
+        L.fd = fd_update_pre(nx, ny, L.fd, E.ffd, TB.cs, CP);
+        L.ht = table_walk_pre(TB.hr, L.ht, E.hd, Rn, CP);
         if (L.ht < E.hd) {
             const int cell = (L.ht * E.ftd + L.td) * E.ffd + L.fd;
             if constexpr (ADV != 0) {
