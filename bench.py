@@ -141,27 +141,23 @@ def percentiles(x) -> dict:
             "max_ms": float(x.max())}
 
 
-def batched_run(lib, args, shape) -> dict:
-    """The batched workload: 64 independent runs (counts log-uniform on [1e4, 2e6],
-    seed 1; body and subsolar longitude drawn per run), icosphere 2562/5120,
-    grid 40x36x72, in one pool; plus the same runs one after another through the
-    plain single-run path."""
-    specs = W.batch_specs(args.batch_runs, seed=1, scale=args.batch_scale)
-    params, surfs = W.build_runs(lib, specs, levels=4, hd=40)
+def batch_once(lib, args, shape, params, surfs, moments: bool) -> dict:
+    """Creates (and autotunes) the batch, launches it args.warmup + args.steps times."""
     pa, sa = W.as_arrays(params, surfs)
-    n = len(specs)
+    n = len(params)
     st = C.c_int(0)
+    flags = 0 if moments else capi.BATCH_DENSITY_ONLY
     t_tune = time.perf_counter()
-    b = lib.dsmc_batch_create(n, pa, sa, 0, C.byref(shape) if shape is not None else None, C.byref(st))
+    b = lib.dsmc_batch_create(n, pa, sa, flags, C.byref(shape) if shape is not None else None, C.byref(st))
     if not b:
         raise RuntimeError(f"dsmc_batch_create failed: {st.value}")
     t_tune = time.perf_counter() - t_tune
-    bufs = [Buffers(p, 5, per_particle=False, moments=True) for p in params]
+    bufs = [Buffers(p, 5, per_particle=False, moments=moments) for p in params]
     arr = (C.POINTER(capi.Result) * n)(*[C.pointer(x.result) for x in bufs])
     for _ in range(args.warmup):
         if lib.dsmc_batch_launch(b, arr):
             raise RuntimeError("batch warmup failed")
-    walls, done = [], np.zeros(n)
+    done = np.zeros(n)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         if lib.dsmc_batch_launch(b, arr):
@@ -171,18 +167,35 @@ def batched_run(lib, args, shape) -> dict:
     lib.dsmc_batch_timing(b, C.byref(w), capi.ptr(done, C.c_double), C.byref(rounds), C.byref(used))
     lib.dsmc_batch_destroy(b)
     steps = sum(int(x.result.total_steps) for x in bufs)
-    out = {"value": steps * args.steps / wall, "unit": "particle-steps/s", "runs": n,
-           "particles": int(sum(s.n for s in specs)), "particle_steps_per_batch": steps,
-           "ms_per_batch": wall / args.steps * 1e3, "rounds": int(rounds.value),
-           "run_finish_ms": percentiles(done), "autotune_s": round(t_tune, 3),
-           "shape": {"block": used.block, "blocks_per_cu": used.blocks_per_cu,
-                     "relaunch_batch": used.relaunch_batch, "step_budget": used.step_budget,
-                     "service_inline": used.service_inline},
-           "graph": "hipGraph of 8 rounds (flight+service+reset) replayed"}
+    return {"value": steps * args.steps / wall, "particle_steps_per_batch": steps,
+            "ms_per_batch": wall / args.steps * 1e3, "rounds": int(rounds.value),
+            "run_finish_ms": percentiles(done), "autotune_s": round(t_tune, 3),
+            "shape": {"block": used.block, "blocks_per_cu": used.blocks_per_cu,
+                      "relaunch_batch": used.relaunch_batch, "step_budget": used.step_budget,
+                      "service_inline": used.service_inline}}
+
+
+def batched_run(lib, args, shape) -> dict:
+    """The batched workload: 64 independent runs (counts log-uniform on [1e4, 2e6],
+    seed 1; body and subsolar longitude drawn per run), icosphere 2562/5120, a
+    40x36x72 FP64 density grid per run, in one pool; the same batch with velocity
+    moments per cell too; and the same runs one after another through the plain
+    single-run path (density grid, default launch shape)."""
+    specs = W.batch_specs(args.batch_runs, seed=1, scale=args.batch_scale)
+    params, surfs = W.build_runs(lib, specs, levels=4, hd=40)
+    main = batch_once(lib, args, shape, params, surfs, moments=False)
+    out = {"value": main.pop("value"), "unit": "particle-steps/s", "runs": len(specs),
+           "particles": int(sum(s.n for s in specs)), "outputs": "density grid per run", **main,
+           "graph": "hipGraph of 8 rounds (flight [+service] + reset) replayed"}
+    if args.batch_moments:
+        wm = batch_once(lib, args, shape, params, surfs, moments=True)
+        out["with_velocity_moments"] = {"value": wm["value"], "ms_per_batch": wm["ms_per_batch"],
+                                        "shape": wm["shape"], "autotune_s": wm["autotune_s"]}
     if args.sequential:
+        st = C.c_int(0)
         t_seq, seq_steps = 0.0, 0
         for p, s in zip(params, surfs):
-            ob = Buffers(p, 5, per_particle=False, moments=True)
+            ob = Buffers(p, 5, per_particle=False, moments=False)
             run = lib.dsmc_run_create(C.byref(p), C.byref(s.surface), C.byref(ob.result), C.byref(st))
             if not run:
                 raise RuntimeError(f"dsmc_run_create failed: {st.value}")
@@ -228,8 +241,9 @@ def collisional_run(lib, args) -> dict:
 def served_run(lib, args, shape, rate: float, seconds: float) -> dict:
     """Poisson request stream (`rate` req/s for `seconds`) of runs, counts
     log-uniform on [1e4, 4e6], body and subsolar longitude drawn per request, 30%
-    collisional (phase 1 + phase 2 at the request's count), through the
-    continuous-batching server; latency = arrival to result ready."""
+    collisional (phase 1 + phase 2 at the request's count), each returning its
+    40x36x72 density grid, through the continuous-batching server; latency =
+    arrival to result ready."""
     stream = W.request_stream(rate, seconds, seed=5, scale=args.serve_scale,
                               frac_collisional=args.serve_collisional)
     specs = [s for _, s in stream]
@@ -241,7 +255,7 @@ def served_run(lib, args, shape, rate: float, seconds: float) -> dict:
     sv = lib.dsmc_server_create(args.serve_budget_gb << 30, 256, 2)      # collisional requests accepted
     if not sv:
         raise RuntimeError("dsmc_server_create failed")
-    bufs = [Buffers(p2s[i] if p2s[i] is not None else p, 5, per_particle=False, moments=True)
+    bufs = [Buffers(p2s[i] if p2s[i] is not None else p, 5, per_particle=False, moments=False)
             for i, p in enumerate(params)]
     tickets, t_sub = [0] * len(stream), [0.0] * len(stream)
     t0 = time.perf_counter()
@@ -333,6 +347,7 @@ def main():
     ap.add_argument("--batch-runs", type=int, default=64)
     ap.add_argument("--batch-scale", type=float, default=1.0, help="multiplies every run's particle count")
     ap.add_argument("--sequential", type=int, default=1, help="also time the batch's runs one by one")
+    ap.add_argument("--batch-moments", type=int, default=1, help="also time the batch with velocity moments")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-seconds", type=float, default=8.0)

@@ -20,6 +20,7 @@ STREAM_BALLISTIC, STREAM_ADVANCED = 1, 2
 FATE_NONE, FATE_ESCAPED, FATE_FROZEN, FATE_PHOTOLYSED, FATE_DEPLETED, FATE_CAPPED = range(6)
 PRESET_COMPAT_BALLISTIC, PRESET_COMPAT_ADVANCED = 0, 1
 PRESET_CERES_PHYSICAL, PRESET_CERES_PHYSICAL_COLL = 2, 3
+BATCH_PER_PARTICLE, BATCH_DENSITY_ONLY = 1, 2
 
 _dp = C.POINTER(C.c_double)
 _i32p = C.POINTER(C.c_int32)

@@ -210,15 +210,19 @@ static int batch_prepare(int nruns, const dsmc_params *params, const dsmc_surfac
 }
 
 extern "C" dsmc_batch *dsmc_batch_create(int nruns, const dsmc_params *params, const dsmc_surface *surfaces,
-                                         int per_particle, const dsmc_launch_shape *tune, int *status)
+                                         int flags, const dsmc_launch_shape *tune, int *status)
 {
     dsmc_batch *b = new dsmc_batch();
     PoolClass cls;
     int64_t total = 0;
-    int rc = batch_prepare(nruns, params, surfaces, &b->preps, &cls, &total);
+    const int per_particle = (flags & DSMC_BATCH_PER_PARTICLE) ? 1 : 0;
+    const int moments = (flags & DSMC_BATCH_DENSITY_ONLY) ? 0 : 1;
+    int rc = (flags & ~(DSMC_BATCH_PER_PARTICLE | DSMC_BATCH_DENSITY_ONLY)) ? DSMC_ERR_ARG : DSMC_OK;
+    if (!rc) rc = batch_prepare(nruns, params, surfaces, &b->preps, &cls, &total);
+    for (auto &pr : b->preps) pr.moments = moments;
     if (!rc) rc = pool_create(&b->P, cls, nruns, total, per_particle, tune);
     if (!rc && !tune) {
-        std::string key = class_key(cls, total, nruns);
+        std::string key = class_key(cls, total, nruns) + (moments ? "" : "/density");
         dsmc_launch_shape s;
         bool hit = false;
         {
@@ -291,11 +295,14 @@ extern "C" int dsmc_run_batch(int nruns, const dsmc_params *params, const dsmc_s
                               dsmc_result *results, const dsmc_launch_shape *tune)
 {
     if (nruns <= 0 || !results) return DSMC_ERR_ARG;
-    int per_particle = 0;
-    for (int i = 0; i < nruns; i++)
-        if (results[i].end_state || results[i].steps || results[i].fate || results[i].hops) per_particle = 1;
+    int flags = DSMC_BATCH_DENSITY_ONLY;
+    for (int i = 0; i < nruns; i++) {
+        if (results[i].end_state || results[i].steps || results[i].fate || results[i].hops)
+            flags |= DSMC_BATCH_PER_PARTICLE;
+        if (results[i].vmom) flags &= ~DSMC_BATCH_DENSITY_ONLY;
+    }
     int rc = 0;
-    dsmc_batch *b = dsmc_batch_create(nruns, params, surfaces, per_particle, tune, &rc);
+    dsmc_batch *b = dsmc_batch_create(nruns, params, surfaces, flags, tune, &rc);
     if (!b) return rc;
     std::vector<dsmc_result *> rp(nruns);
     for (int i = 0; i < nruns; i++) rp[i] = &results[i];

@@ -38,6 +38,7 @@ struct RunDev {
     unsigned int *host_ctl = nullptr;     /* pinned readback of qctl */
     int rounds = 0;
     double flight_ms = 0.0;
+    int short_hops = 0;                   /* physical: landings every few dozen steps */
     ~RunDev();
 };
 
@@ -47,7 +48,7 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
               RunDev *rd, hipStream_t st);
 int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int timed = 0);
 int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st);
-dsmc_launch_shape default_shape(const DsmcEnv &E);
+dsmc_launch_shape default_shape(const DsmcEnv &E, int short_hops);
 void number_density(const dsmc_params *p1, double wsum, const int64_t *vcount, double *ndens);
 bool shape_ok(const dsmc_launch_shape &s);
 

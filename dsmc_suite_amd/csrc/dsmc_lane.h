@@ -709,15 +709,23 @@ DSMC_HD double face_margin(const DsmcEnv &E, int f, double dx, double dy, double
 }
 
 /* Finest icosphere face containing direction d (largest minimum edge margin at
- * every level of the subdivision hierarchy, lowest index on ties). */
+ * every level of the subdivision hierarchy, lowest index on ties).  The margins
+ * of a group of faces are computed before any of them is compared, so the
+ * group's normals load together (one memory round trip per group: 5 for the 20
+ * top faces, 1 per level below); the comparisons run in index order as a
+ * one-at-a-time scan would make them. */
 DSMC_HD int locate_face(const DsmcEnv &E, double dx, double dy, double dz)
 {
     int best = 0;
     double bm = face_margin(E, 0, dx, dy, dz);
 #pragma unroll 1
-    for (int f = 1; f < 20; f++) {
-        double m = face_margin(E, f, dx, dy, dz);
-        if (m > bm) { bm = m; best = f; }
+    for (int f0 = 1; f0 < 20; f0 += 4) {
+        double m[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) m[j] = f0 + j < 20 ? face_margin(E, f0 + j, dx, dy, dz) : 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (f0 + j < 20 && m[j] > bm) { bm = m[j]; best = f0 + j; }
     }
     /* level l starts at face 20 (4^l - 1) / 3 of the all-levels arrays */
     int off = 0, nl = 20;
@@ -725,13 +733,14 @@ DSMC_HD int locate_face(const DsmcEnv &E, double dx, double dy, double dz)
     for (int l = 1; l <= E.levels; l++) {
         off += nl;
         nl *= 4;
-        int base = 4 * best, pick = base;
-        double pm = face_margin(E, off + base, dx, dy, dz);
-#pragma unroll 1
-        for (int j = 1; j < 4; j++) {
-            double m = face_margin(E, off + base + j, dx, dy, dz);
-            if (m > pm) { pm = m; pick = base + j; }
-        }
+        const int base = 4 * best;
+        const double m0 = face_margin(E, off + base, dx, dy, dz), m1 = face_margin(E, off + base + 1, dx, dy, dz);
+        const double m2 = face_margin(E, off + base + 2, dx, dy, dz), m3 = face_margin(E, off + base + 3, dx, dy, dz);
+        int pick = base;
+        double pm = m0;
+        if (m1 > pm) { pm = m1; pick = base + 1; }
+        if (m2 > pm) { pm = m2; pick = base + 2; }
+        if (m3 > pm) { pm = m3; pick = base + 3; }
         best = pick;
     }
     return best;

@@ -120,6 +120,13 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
         E.vert = (const double *)rd->vert.ptr;
         E.faces = (const int32_t *)rd->faces.ptr;
         E.face_T = (const double *)rd->face_T.ptr;
+        /* short hops: the body holds its gas close (escape speed more than twice
+         * the mean thermal speed at the hottest face), so a particle lands every
+         * few dozen steps and the landing service belongs in the flight kernel */
+        double tmax = 0.0;
+        for (size_t f = 0; f < T.nfine; f++) tmax = s->face_T[f] > tmax ? s->face_T[f] : tmax;
+        const double v_th = sqrt(2.0 * p->k_B * tmax / p->mmass);
+        rd->short_hops = sqrt(2.0 * p->G * p->M / p->radius) > 2.0 * v_th ? 1 : 0;
     }
 #undef UP
     if ((rc = rd->density.zeros(cells * sizeof(double), st))) return rc;
@@ -211,7 +218,7 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
     int cus = 256;
     int rc = device_props(&cus);
     if (rc) return rc;
-    dsmc_launch_shape sh = shape ? *shape : default_shape(rd->E);
+    dsmc_launch_shape sh = shape ? *shape : default_shape(rd->E, rd->short_hops);
     if (!shape_ok(sh)) return DSMC_ERR_ARG;
     /* physical collisional runs always service inline (collisions are service events) */
     const bool inline_svc = rd->E.mode == DSMC_MODE_PHYSICAL && (sh.service_inline || rd->E.advanced);
@@ -220,7 +227,8 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
          * kernel's registers or LDS do not allow them */
         int occ = dsmc_flight_occupancy(&rd->E, inline_svc ? 1 : 0, sh.block);
         if (occ > 0 && occ < sh.blocks_per_cu) sh.blocks_per_cu = occ;
-        if (inline_svc && sh.step_budget > 1024) sh.step_budget = 1024;
+        /* collisional runs (inline by necessity): 1024-step launches */
+        if (rd->E.advanced && inline_svc && sh.step_budget > 1024) sh.step_budget = 1024;
     }
     const int64_t n = rd->E.nparticles;
     DsmcQueues Q;
@@ -287,16 +295,32 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
     return DSMC_OK;
 }
 
-dsmc_launch_shape default_shape(const DsmcEnv &E)
+dsmc_launch_shape default_shape(const DsmcEnv &E, int short_hops)
 {
     dsmc_launch_shape s;
     s.block = 256;
-    /* physical, Ceres 1e7 on MI355X: 5 blocks per CU and a 4096-step budget
-     * 1.22e11 particle-steps/s, 4 blocks and 1024 steps 1.13e11 */
-    s.blocks_per_cu = E.mode == 0 ? 4 : 5;
-    s.relaunch_batch = E.mode == 0 ? 8 : 16;
-    s.step_budget = 4096;
-    s.service_inline = 0;     /* long-hop single runs: the lean flight kernel (autotuning may pick 1) */
+    if (E.mode == 0) {
+        s.blocks_per_cu = 4;
+        s.relaunch_batch = 8;
+        s.step_budget = 4096;
+        s.service_inline = 0;
+    } else if (short_hops) {
+        /* Moon / Mercury / Europa single runs of 1e4-2e6 particles on MI355X:
+         * landing service inline, 48-lane batches, 2048-step budget 2.3e10
+         * particle-steps/s; 32-lane batches 1.5e10; the split kernels 5e9 */
+        s.blocks_per_cu = 4;
+        s.relaunch_batch = 48;
+        s.step_budget = 2048;
+        s.service_inline = 1;
+    } else {
+        /* long hops (Ceres): the lean flight kernel.  1e7 particles: 5 blocks per
+         * CU and a 4096-step budget 1.22e11 particle-steps/s (4 blocks, 1024
+         * steps: 1.13e11); runs of 1e4-2e6 particles do best with 1024 steps */
+        s.blocks_per_cu = 5;
+        s.relaunch_batch = 16;
+        s.step_budget = E.nparticles >= 4000000 ? 4096 : 1024;
+        s.service_inline = 0;
+    }
     return s;
 }
 

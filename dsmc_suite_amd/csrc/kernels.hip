@@ -34,17 +34,41 @@
 static_assert(sizeof(DsmcRec) == 256, "particle record must stay 256 bytes");
 
 /* Deposits straight into the device grid (global FP64 atomics). */
+/* Atomics on device memory through global (not generic) addresses.  A generic
+ * ("flat") memory instruction also counts against the wavefront's LDS wait
+ * counter, so an outstanding flat atomic - a deposit to a contended cell takes
+ * microseconds - would hold up the next LDS table read of the flight loop; a
+ * global one does not.  Every pointer given to these is device memory. */
+#define DSMC_GLOBAL __attribute__((address_space(1)))
+__device__ static inline void gadd(double *p, double v)
+{
+    __hip_atomic_fetch_add((DSMC_GLOBAL double *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ static inline void gadd(unsigned long long *p, unsigned long long v)
+{
+    __hip_atomic_fetch_add((DSMC_GLOBAL unsigned long long *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ static inline int32_t gclaim(int32_t *p)
+{
+    return __hip_atomic_fetch_add((DSMC_GLOBAL int32_t *)p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ static inline unsigned long long gclaim(unsigned long long *p)
+{
+    return __hip_atomic_fetch_add((DSMC_GLOBAL unsigned long long *)p, 1ull, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <class EnvT>
 __device__ static inline void grid_deposit(const EnvT &E, int cell, double w, double vx, double vy,
                                            double vz, int n)
 {
-    if (!E.density_from_count) atomicAdd(&E.density[cell], w);
+    if (!E.density_from_count) gadd(&E.density[cell], w);
     if (E.vmom) {
-        atomicAdd(&E.vmom[(size_t)cell * 3 + 0], vx);
-        atomicAdd(&E.vmom[(size_t)cell * 3 + 1], vy);
-        atomicAdd(&E.vmom[(size_t)cell * 3 + 2], vz);
+        gadd(&E.vmom[(size_t)cell * 3 + 0], vx);
+        gadd(&E.vmom[(size_t)cell * 3 + 1], vy);
+        gadd(&E.vmom[(size_t)cell * 3 + 2], vz);
     }
-    if (E.vcount) atomicAdd(&E.vcount[cell], (unsigned long long)n);
+    if (E.vcount) gadd(&E.vcount[cell], (unsigned long long)n);
 }
 
 /* The speed histogram of a visit (physical runs that keep one). */
@@ -52,7 +76,7 @@ template <class EnvT>
 __device__ static inline void visit_hist(const EnvT &E, int cell, double vx, double vy, double vz, int n)
 {
     unsigned long long *hb = vhist_bin(E, cell, vx, vy, vz, n);
-    if (hb) atomicAdd(hb, (unsigned long long)n);
+    if (hb) gadd(hb, (unsigned long long)n);
 }
 
 struct DsmcOpsDev {
@@ -63,10 +87,10 @@ struct DsmcOpsDev {
         visit_hist(E, cell, vx, vy, vz, n);
         grid_deposit(E, cell, w, vx, vy, vz, n);
     }
-    __device__ static inline void add_f64(double *p, double v) { atomicAdd(p, v); }
-    __device__ static inline void add_u64(unsigned long long *p, unsigned long long v) { atomicAdd(p, v); }
-    __device__ static inline int32_t claim_i32(int32_t *p) { return atomicAdd(p, 1); }
-    __device__ static inline unsigned long long claim_u64(unsigned long long *p) { return atomicAdd(p, 1ull); }
+    __device__ static inline void add_f64(double *p, double v) { gadd(p, v); }
+    __device__ static inline void add_u64(unsigned long long *p, unsigned long long v) { gadd(p, v); }
+    __device__ static inline int32_t claim_i32(int32_t *p) { return gclaim(p); }
+    __device__ static inline unsigned long long claim_u64(unsigned long long *p) { return gclaim(p); }
 };
 
 /* LDS-privatised deposit of the flight kernel: a per-block table of DEP_SLOTS grid
@@ -98,10 +122,10 @@ struct DsmcOpsLds {
         if (E.vmom) { atomicAdd(&dep_vx[s], vx); atomicAdd(&dep_vy[s], vy); atomicAdd(&dep_vz[s], vz); }
         if (E.vcount) atomicAdd(&dep_n[s], n);
     }
-    __device__ static inline void add_f64(double *p, double v) { atomicAdd(p, v); }
-    __device__ static inline void add_u64(unsigned long long *p, unsigned long long v) { atomicAdd(p, v); }
-    __device__ static inline int32_t claim_i32(int32_t *p) { return atomicAdd(p, 1); }
-    __device__ static inline unsigned long long claim_u64(unsigned long long *p) { return atomicAdd(p, 1ull); }
+    __device__ static inline void add_f64(double *p, double v) { gadd(p, v); }
+    __device__ static inline void add_u64(unsigned long long *p, unsigned long long v) { gadd(p, v); }
+    __device__ static inline int32_t claim_i32(int32_t *p) { return gclaim(p); }
+    __device__ static inline unsigned long long claim_u64(unsigned long long *p) { return gclaim(p); }
 };
 
 __device__ static inline void dep_init()
@@ -155,7 +179,7 @@ __device__ static inline void counts_flush_wave(DsmcCounts &cnt, unsigned long l
             unsigned long long x = mine ? v[i] : 0ull;
 #pragma unroll
             for (int o = 32; o >= 1; o >>= 1) x += (unsigned long long)__shfl_xor((long long)x, o);
-            if ((int)__lane_id() == leader && x) atomicAdd(&lead[i], x);
+            if ((int)__lane_id() == leader && x) gadd(&lead[i], x);
         }
         active = active && !mine;
     }
@@ -347,9 +371,14 @@ pool_flight_kernel(DsmcPoolDev P, DsmcQueues Q, int p, int refill_batch, int64_t
  * run's environment is copied from device memory once, at the top (its fields
  * load together instead of one dependent load per use). */
 template <class Ops, int ADV>
-__device__ __attribute__((noinline)) void phys_service(DsmcLane &T, const DsmcEnv *Ep, DsmcCounts &cnt)
+__device__ __attribute__((noinline)) void phys_service(DsmcLane &T, const DsmcEnv *Ep, DsmcCounts &cnt,
+                                                       const DsmcCellTabs TB)
 {
-    const DsmcEnv E = *Ep;
+    DsmcEnv E = *Ep;
+    /* the cell walks of a relaunch read the kernel's LDS copies of the tables */
+    E.tq = TB.tq;
+    E.cs = TB.cs;
+    E.hr = TB.hr;
     lane_service<Ops, 1, ADV>(T, E, cnt);
 }
 
@@ -397,8 +426,11 @@ __device__ __forceinline__ void lane_collide_lean(DsmcLane &L, const DsmcEnv *Ep
  * particles of different runs (step constants in registers, every run's radial
  * thresholds in LDS); otherwise one run's environment is the kernel argument and
  * deposits go through the LDS slot table. */
+/* Pools: up to 168 VGPRs (3 waves per SIMD), so that the lanes' per-run step
+ * constants do not spill inside the flight loop (a reload waits behind the
+ * wavefront's outstanding deposit atomics); single runs: 128 (4 waves). */
 template <bool POOL, int ADV>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(POOL ? 3 : 4)))
 phys_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcPoolDev P, DsmcRec *rec, DsmcQueues Q, int p, int svc_batch,
             int64_t step_budget)
 {
@@ -478,8 +510,8 @@ phys_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcPoolDev P, DsmcRec *rec, DsmcQueu
         const bool svc = L.phase == LANE_LANDED || L.phase == LANE_ENDED || L.phase == LANE_COLLIDE;
         if (svc) {
             DsmcLane T = L;
-            if (POOL) phys_service<DsmcOpsDev, ADV>(T, P.envs + run, cnt);
-            else phys_service<DsmcOpsLds, ADV>(T, Ed, cnt);
+            if (POOL) phys_service<DsmcOpsDev, ADV>(T, P.envs + run, cnt, TB);
+            else phys_service<DsmcOpsLds, ADV>(T, Ed, cnt, TB);
             L = T;
         }
         const bool fin = svc && L.phase == LANE_DONE;

@@ -99,6 +99,7 @@ struct PoolPrep {
     std::vector<double> vert;
     std::vector<int32_t> faces;
     uint64_t geo_key = 0;
+    int moments = 1;           /* velocity moments per cell (3 more atomics per cell visit) */
 };
 
 int pool_class_of(const dsmc_params *p, const dsmc_surface *s, PoolClass *c);

@@ -459,7 +459,7 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out, int hist, 
     E.density = nullptr;
     E.density_from_count = 1;
     E.vcount = (unsigned long long *)P->vcount.ptr + (size_t)g * cells;
-    E.vmom = (double *)P->vmom.ptr + (size_t)g * cells * 3;
+    E.vmom = pr.moments ? (double *)P->vmom.ptr + (size_t)g * cells * 3 : nullptr;
     E.horizon = (double *)P->horizon.ptr + (size_t)g * P->cls.ftd * P->cls.ffd;
     E.counters = (unsigned long long *)P->counters.ptr + (size_t)g * 8;
     E.ttime = nullptr;
@@ -503,7 +503,7 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out, int hist, 
     HIP_OK(hipMemcpyAsync((DsmcStepEnv *)P->steps_d.ptr + g, &P->steps_h[g], sizeof(DsmcStepEnv),
                           hipMemcpyHostToDevice, st));
     HIP_OK(dsmc_launch_fill0(E.vcount, cells, st));
-    HIP_OK(dsmc_launch_fill0(E.vmom, cells * 3, st));
+    if (E.vmom) HIP_OK(dsmc_launch_fill0(E.vmom, cells * 3, st));
     HIP_OK(dsmc_launch_fill0(E.horizon, (size_t)P->cls.ftd * P->cls.ffd, st));
     HIP_OK(dsmc_launch_fill0(E.counters, 8, st));
     if (E.vhist) HIP_OK(dsmc_launch_fill0(E.vhist, cells * P->hist_bins, st));
@@ -586,7 +586,7 @@ int pool_collect(Pool *P, int g, dsmc_result *r)
     char *o = (char *)P->out_h;
     unsigned long long cnt[8];
     HIP_OK(hipMemcpyAsync(o, E.vcount, cells * 8, hipMemcpyDeviceToHost, cs));
-    HIP_OK(hipMemcpyAsync(o + cells * 8, E.vmom, cells * 24, hipMemcpyDeviceToHost, cs));
+    if (E.vmom) HIP_OK(hipMemcpyAsync(o + cells * 8, E.vmom, cells * 24, hipMemcpyDeviceToHost, cs));
     HIP_OK(hipMemcpyAsync(o + cells * 32, E.horizon, nh * 8, hipMemcpyDeviceToHost, cs));
     HIP_OK(hipMemcpyAsync(o + cells * 32 + nh * 8, E.counters, sizeof cnt, hipMemcpyDeviceToHost, cs));
     HIP_OK(hipStreamSynchronize(cs));
@@ -594,7 +594,10 @@ int pool_collect(Pool *P, int g, dsmc_result *r)
     const int64_t *vc = (const int64_t *)o;
     if (r->vcount) memcpy(r->vcount, vc, cells * 8);
     if (r->density) for (size_t i = 0; i < cells; i++) r->density[i] = (double)vc[i];
-    if (r->vmom) memcpy(r->vmom, o + cells * 8, cells * 24);
+    if (r->vmom) {
+        if (E.vmom) memcpy(r->vmom, o + cells * 8, cells * 24);
+        else memset(r->vmom, 0, cells * 24);       /* a density-only run */
+    }
     if (r->horizon) memcpy(r->horizon, o + cells * 32, nh * 8);
     const int64_t n = R.n;
     if (P->per_particle) {

@@ -153,8 +153,12 @@ static int submit(dsmc_server *sv, const dsmc_params *p, const dsmc_params *p2, 
         sv->reqs.erase(t);
         return rc;
     }
+    /* velocity moments only for requests that ask for them (and never for a
+     * collisional request's phase 1, which hands over counts and histograms) */
+    q.prep.moments = (!p2 && r->vmom) ? 1 : 0;
     if (p2) {
         q.collisional = 1;
+        q.prep2.moments = r->vmom ? 1 : 0;
         q.vcount1.assign((size_t)p->ftd * p->ffd * p->hd, 0);
     }
     q.r = r;

@@ -278,11 +278,18 @@ int dsmc_run_batch(int nruns, const dsmc_params *params, const dsmc_surface *sur
                    dsmc_result *results, const dsmc_launch_shape *tune);
 
 /* Staged form: inputs prepared and device memory allocated once (and the shape
- * tuned), then launched any number of times.  per_particle: keep per-particle
- * outputs (end_state, steps, fate, hops; 64 B per particle). */
+ * tuned), then launched any number of times.  flags: DSMC_BATCH_PER_PARTICLE
+ * keeps per-particle outputs (end_state, steps, fate, hops; 64 B per particle);
+ * DSMC_BATCH_DENSITY_ONLY skips the velocity moments (one atomic per cell visit
+ * instead of four; results' vmom, if given, are zero).  dsmc_run_batch is
+ * density-only when no result asks for vmom. */
 typedef struct dsmc_batch dsmc_batch;
+enum dsmc_batch_flags {
+    DSMC_BATCH_PER_PARTICLE = 1,
+    DSMC_BATCH_DENSITY_ONLY = 2
+};
 dsmc_batch *dsmc_batch_create(int nruns, const dsmc_params *params, const dsmc_surface *surfaces,
-                              int per_particle, const dsmc_launch_shape *tune, int *status);
+                              int flags, const dsmc_launch_shape *tune, int *status);
 /* Runs every run to completion; results[i] (or results == NULL) receive run i. */
 int dsmc_batch_launch(dsmc_batch *b, dsmc_result *const *results);
 /* Host wall time of the last launch, each run's finish time, rounds, shape. */
@@ -316,7 +323,8 @@ enum dsmc_serve_flags {
 dsmc_server *dsmc_server_create(size_t hbm_budget_bytes, int max_inflight, int flags);
 int dsmc_server_set_shape(dsmc_server *sv, const dsmc_launch_shape *shape);   /* while idle */
 /* r must stay valid until the request completes; r->elapsed_ms receives the time
- * from submission to completion. */
+ * from submission to completion.  Velocity moments are computed only for
+ * requests whose r->vmom is set. */
 int dsmc_server_submit(dsmc_server *sv, const dsmc_params *p, const dsmc_surface *s,
                        dsmc_result *r, uint64_t *ticket);
 /* A physical collisional request: phase 1 (p1, speed histograms kept), then

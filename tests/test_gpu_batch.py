@@ -121,6 +121,34 @@ def test_staged_batch_relaunch(lib):
         assert x.result.total_steps == y.result.total_steps
 
 
+def test_batch_density_only_equals_full(lib):
+    """A density-only batch (one deposit atomic per cell visit) gives the full
+    batch's per-particle results, counts, horizon and fate counters bit for bit;
+    velocity moments it was not asked for come back zero."""
+    sp = specs()
+    params, surfs = W.build_runs(lib, sp, levels=LEVELS, hd=HD)
+    pa, sa = W.as_arrays(params, surfs)
+    shape = capi.LaunchShape(256, 4, 16, 1024, 1)
+    st = C.c_int(0)
+    outs = []
+    for flags in (capi.BATCH_PER_PARTICLE, capi.BATCH_PER_PARTICLE | capi.BATCH_DENSITY_ONLY):
+        print(f"batch flags {flags}", flush=True)
+        b = lib.dsmc_batch_create(len(sp), pa, sa, flags, C.byref(shape), C.byref(st))
+        assert b, st.value
+        bufs = [Buffers(p, 5, per_particle=True, moments=True) for p in params]
+        for x in bufs:
+            x.vmom[:] = 1.0                     # must be overwritten
+        arr = (C.POINTER(capi.Result) * len(sp))(*[C.pointer(x.result) for x in bufs])
+        assert lib.dsmc_batch_launch(b, arr) == 0
+        lib.dsmc_batch_destroy(b)
+        outs.append(bufs)
+    for i, (x, y) in enumerate(zip(*outs)):
+        for f in ("end_state", "steps", "fate", "hops", "vcount", "density", "horizon"):
+            assert np.array_equal(getattr(x, f), getattr(y, f)), (i, f)
+        assert {k: int(getattr(x.result, k)) for k in COUNTERS} == {k: int(getattr(y.result, k)) for k in COUNTERS}
+        assert np.any(x.vmom != 0.0) and np.all(y.vmom == 0.0), i
+
+
 def test_server_alone_equals_mix(lib):
     """Continuous batching: run A alone, then A admitted while other runs fly."""
     sp = specs()
