@@ -119,3 +119,76 @@ def run_oracle(lib, p: Params, surf: SurfaceArrays, rows=5, nthreads=1, rng=ORNG
 def c_fmt(fmt: str, vals) -> str:
     """printf-style formatting of a sequence (Python's % matches C for these specs)."""
     return "".join(fmt % v for v in vals)
+
+
+# ---------------------------------------------------------------- statistics
+# Coarse cells of a [ht][td][fd] grid (36 x 72 angular cells): the surface shell
+# in 12 latitude bands x 12 longitude sectors (15 x 30 degrees; most of the gas
+# of the compat runs never leaves it), the shells above in groups that double in
+# depth, 6 x 6 (30 x 60 degrees).
+COARSE_HT = (0, 1, 2, 3, 4, 8, 16, 32, 100)
+
+
+def coarse_density(den: np.ndarray, hd: int, ftd: int, ffd: int, ht_edges=COARSE_HT) -> np.ndarray:
+    """Sums of a [ht][td][fd] density field over coarse cells, flattened."""
+    d = np.asarray(den, dtype=np.float64).reshape(hd, ftd, ffd)
+    surface = d[0].reshape(12, ftd // 12, 12, ffd // 12).sum(axis=(1, 3)).ravel()
+    c = d.reshape(hd, 6, ftd // 6, 6, ffd // 6).sum(axis=(2, 4))
+    edges = [e for e in ht_edges if 1 <= e <= hd]
+    if edges[-1] != hd:
+        edges.append(hd)
+    above = [c[a:b].sum(axis=0).ravel() for a, b in zip(edges[:-1], edges[1:])]
+    return np.concatenate([surface, *above])
+
+
+def two_sample_chi2(a: np.ndarray, b: np.ndarray, edges=None) -> tuple[float, int, float]:
+    """Two-sample chi-squared test of two samples of one quantity: X^2 =
+    sum (sqrt(Nb/Na) a_k - sqrt(Na/Nb) b_k)^2 / (a_k + b_k) over bins, with
+    (bins - 1) degrees of freedom.  Bins: `edges` (right-open), or the pooled
+    quantiles with ~25 values of the smaller sample per bin (4 to 16 bins); bins
+    with fewer than 10 pooled values are merged into their neighbour.  Returns
+    (X^2, dof, p)."""
+    from scipy.stats import chi2
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    pooled = np.concatenate([a, b])
+    if edges is None:
+        nbins = int(np.clip(min(len(a), len(b)) // 25, 4, 16))
+        edges = np.unique(np.quantile(pooled, np.linspace(0, 1, nbins + 1)))
+        edges[-1] = np.nextafter(edges[-1], np.inf)      # the largest value in the last bin
+    edges = np.asarray(edges, np.float64)
+    ca = np.histogram(a, edges)[0].astype(np.float64)
+    cb = np.histogram(b, edges)[0].astype(np.float64)
+    # merge sparse bins
+    ma, mb, acc_a, acc_b = [], [], 0.0, 0.0
+    for x, y in zip(ca, cb):
+        acc_a += x
+        acc_b += y
+        if acc_a + acc_b >= 10:
+            ma.append(acc_a); mb.append(acc_b)
+            acc_a = acc_b = 0.0
+    if acc_a + acc_b > 0:
+        if ma:
+            ma[-1] += acc_a; mb[-1] += acc_b
+        else:
+            ma.append(acc_a); mb.append(acc_b)
+    ca, cb = np.array(ma), np.array(mb)
+    na, nb = ca.sum(), cb.sum()
+    x2 = float(np.sum((np.sqrt(nb / na) * ca - np.sqrt(na / nb) * cb) ** 2 / (ca + cb)))
+    dof = len(ca) - 1
+    return x2, dof, float(chi2.sf(x2, dof)) if dof > 0 else 1.0
+
+
+def per_cell_t(ref: np.ndarray, reps: np.ndarray, frac_of_max: float = 1e-3) -> tuple[float, int]:
+    """One reference sample per coarse cell against R replicas of the same size:
+    t_c = (ref_c - mean_c) / (s_c sqrt(1 + 1/R)) ~ Student t(R - 1) per cell;
+    returns the smallest two-sided p over the cells whose replica mean exceeds
+    frac_of_max of the largest one, Bonferroni-adjusted (times the number of
+    cells: valid whatever the correlation between cells), and that number."""
+    from scipy.stats import t as student
+    reps = np.asarray(reps, np.float64)
+    R = reps.shape[0]
+    m, s = reps.mean(axis=0), reps.std(axis=0, ddof=1)
+    keep = (m > frac_of_max * m.max()) & (s > 0)
+    tc = (np.asarray(ref)[keep] - m[keep]) / (s[keep] * math.sqrt(1.0 + 1.0 / R))
+    p = 2.0 * student.sf(np.abs(tc), R - 1)
+    return float(min(1.0, p.min() * keep.sum())), int(keep.sum())
