@@ -206,11 +206,15 @@ DSMC_HD dsmc_u32x4 lane_block(uint64_t seed, uint64_t pid, uint32_t stream, uint
     return dsmc_rng_block(seed, pid, stream, b);
 }
 
+/* INL: the Philox rounds inline (a caller that is itself out of line and draws
+ * several blocks in a row: no call per block); same words either way. */
+template <bool INL = false>
 DSMC_HD uint32_t lane_word(DsmcLane &L, const DsmcEnv &E)
 {
     uint64_t b = L.k >> 2;
     if (b != L.blk) {
-        dsmc_u32x4 r = lane_block(E.seed, (uint64_t)L.pid, E.stream, b);
+        dsmc_u32x4 r = INL ? dsmc_rng_block(E.seed, (uint64_t)L.pid, E.stream, b)
+                           : lane_block(E.seed, (uint64_t)L.pid, E.stream, b);
         L.rb0 = r.w[0]; L.rb1 = r.w[1]; L.rb2 = r.w[2]; L.rb3 = r.w[3];
         L.blk = b;
     }
@@ -219,10 +223,11 @@ DSMC_HD uint32_t lane_word(DsmcLane &L, const DsmcEnv &E)
     return j == 0 ? L.rb0 : j == 1 ? L.rb1 : j == 2 ? L.rb2 : L.rb3;
 }
 DSMC_HD int lane_draw(DsmcLane &L, const DsmcEnv &E) { return (int)(lane_word(L, E) >> 1); }
+template <bool INL = false>
 DSMC_HD double lane_u01(DsmcLane &L, const DsmcEnv &E)
 {
-    uint32_t hi = lane_word(L, E);
-    uint32_t lo = lane_word(L, E);
+    uint32_t hi = lane_word<INL>(L, E);
+    uint32_t lo = lane_word<INL>(L, E);
     return dsmc_u01_from_words(hi, lo);
 }
 DSMC_HD double lane_urand(DsmcLane &L, const DsmcEnv &E)   /* (double)rand_r / RAND_MAX */
@@ -818,29 +823,31 @@ DSMC_HD double sample_speed_phys(const double *cg, int deg, double min_v, double
 
 /* Elastic hard-sphere collision with a partner of the cell's speed distribution
  * moving in an isotropic direction; equal masses, isotropic scattering in the
- * centre-of-mass frame.  Then a new collision depth threshold. */
+ * centre-of-mass frame.  Then a new collision depth threshold.  INL: its twelve
+ * random words from inline Philox rounds (for the out-of-line collision path). */
+template <bool INL = false>
 DSMC_HD void lane_collide(DsmcLane &L, const DsmcEnv &E)
 {
     if (L.ht >= E.hd) { L.real_col_p = 0.0; L.phase = LANE_FLIGHT; return; }   /* cannot happen: depth grows in-grid */
     const int c = (L.ht * E.ftd + L.td) * E.ffd + L.fd;
     const int deg = E.fit_deg;
-    double u = lane_u01(L, E);
+    double u = lane_u01<INL>(L, E);
     double vv2 = sample_speed_phys(E.coeffs + (size_t)c * (deg + 1), deg, E.pmin[c], E.pmax[c], u);
-    double ct = 2.0 * lane_u01(L, E) - 1.0, st = sqrt(1.0 - ct * ct);
+    double ct = 2.0 * lane_u01<INL>(L, E) - 1.0, st = sqrt(1.0 - ct * ct);
     double sp, cp;
-    dm_sincos(2.0 * M_PI * lane_u01(L, E), &sp, &cp);
+    dm_sincos(2.0 * M_PI * lane_u01<INL>(L, E), &sp, &cp);
     double v2x = vv2 * st * cp, v2y = vv2 * st * sp, v2z = vv2 * ct;
     double gx = L.vx - v2x, gy = L.vy - v2y, gz = L.vz - v2z;
     double g = sqrt(gx * gx + gy * gy + gz * gz);
-    double ct2 = 2.0 * lane_u01(L, E) - 1.0, st2 = sqrt(1.0 - ct2 * ct2);
+    double ct2 = 2.0 * lane_u01<INL>(L, E) - 1.0, st2 = sqrt(1.0 - ct2 * ct2);
     double sp2, cp2;
-    dm_sincos(2.0 * M_PI * lane_u01(L, E), &sp2, &cp2);
+    dm_sincos(2.0 * M_PI * lane_u01<INL>(L, E), &sp2, &cp2);
     L.vx = 0.5 * (L.vx + v2x) + 0.5 * g * st2 * cp2;
     L.vy = 0.5 * (L.vy + v2y) + 0.5 * g * st2 * sp2;
     L.vz = 0.5 * (L.vz + v2z) + 0.5 * g * ct2;
     L.num_col++;
     L.real_col_p = 0.0;
-    L.col_p = -dm_log(lane_u01(L, E));
+    L.col_p = -dm_log(lane_u01<INL>(L, E));
     L.phase = LANE_FLIGHT;
 }
 

@@ -399,7 +399,7 @@ __device__ __attribute__((noinline)) CollideOut phys_collide(double vx, double v
     T.k = k; T.pid = pid; T.blk = ~0ull;
     T.ht = ht; T.td = td; T.fd = fd;
     T.num_col = 0;
-    lane_collide(T, E);
+    lane_collide<true>(T, E);
     CollideOut o = {T.vx, T.vy, T.vz, T.col_p, T.k, T.num_col};
     return o;
 }
