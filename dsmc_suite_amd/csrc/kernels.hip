@@ -265,9 +265,14 @@ flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch, in
         }
         /* step until enough lanes have ended their hop */
         for (;;) {
-            if (L.phase == LANE_FLIGHT) {
-                lane_step<DsmcOpsLds, MODE, ADV>(L, E, TB);
-                if (L.phase == LANE_FLIGHT && L.steps >= budget_end) L.phase = LANE_IDLE + 8;
+            /* two steps per vote of the wavefront (a lane whose hop ended in the
+             * first one idles through the second): +1.6% on the Ceres 1e7 run */
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                if (L.phase == LANE_FLIGHT) {
+                    lane_step<DsmcOpsLds, MODE, ADV>(L, E, TB);
+                    if (L.phase == LANE_FLIGHT && L.steps >= budget_end) L.phase = LANE_IDLE + 8;
+                }
             }
             uint64_t ended = __ballot(L.phase != LANE_FLIGHT && L.phase != LANE_IDLE);
             if (ended && (__popcll(ended | idle) >= refill_batch || __ballot(L.phase == LANE_FLIGHT) == 0)) break;
@@ -489,10 +494,15 @@ phys_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcPoolDev P, DsmcRec *rec, DsmcQueu
         }
         if (POOL) { S.ftd = P.ftd; S.ffd = P.ffd; S.hd = P.hd; }
         for (;;) {
-            if (L.phase == LANE_FLIGHT) {
-                if (POOL) lane_step<DsmcOpsDev, 1, ADV>(L, S, TB);
-                else lane_step<DsmcOpsLds, 1, ADV>(L, E0, TB);
-                if (L.phase == LANE_FLIGHT && L.steps >= budget_end) L.phase = LANE_IDLE + 8;
+            /* two steps per vote, as in flight_kernel (+2-4% on pools, single
+             * short-hop and collisional runs) */
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                if (L.phase == LANE_FLIGHT) {
+                    if (POOL) lane_step<DsmcOpsDev, 1, ADV>(L, S, TB);
+                    else lane_step<DsmcOpsLds, 1, ADV>(L, E0, TB);
+                    if (L.phase == LANE_FLIGHT && L.steps >= budget_end) L.phase = LANE_IDLE + 8;
+                }
             }
             /* collisions in a dense gas come every few steps: once COLLIDE_BATCH
              * lanes wait for one they take the lean path here, without ending
