@@ -291,12 +291,16 @@ DSMC_HD int fd_update(double x, double y, int fd, int ffd, const double *cs)
     return fd_exact(x, y, ffd);
 }
 
-/* Latitude index from (z, Ri): kept while z/Ri is farther than 1e-12 (relative)
- * from both thresholds of cell td (no division), else walked on q = z/Ri. */
+/* Latitude index from (z, Ri): cell td, or a neighbour, while z/Ri is farther
+ * than 1e-12 (relative) from both of its thresholds (no division: such a margin
+ * leaves the walk on q = z/Ri no other answer), else walked on q = z/Ri. */
 DSMC_HD int td_update(double z, double Ri, int td, int ftd, const double *tq)
 {
     double eps = 1e-12 * Ri;
-    if (z - tq[td] * Ri > eps && tq[td + 1] * Ri - z > eps) return td;
+    const double lo = tq[td] * Ri, hi = tq[td + 1] * Ri;
+    if (z - lo > eps && hi - z > eps) return td;
+    if (td > 0 && z - tq[td - 1] * Ri > eps && lo - z > eps) return td - 1;
+    if (td < ftd - 1 && z - hi > eps && tq[td + 2] * Ri - z > eps) return td + 1;
     double q = z / Ri;
     while (td > 0 && q < tq[td]) td--;
     while (td < ftd - 1 && q >= tq[td + 1]) td++;
