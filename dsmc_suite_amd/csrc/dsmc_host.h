@@ -28,7 +28,7 @@ struct DevBuf {
 struct RunDev {
     DsmcEnv E;
     HostTables T;
-    DevBuf tq, hr, cs, vt_speeds, vt_count, Tsurf, cdf, enorm, fnorm, vert, faces, face_T;
+    DevBuf tq, hr, cs, vt_speeds, vt_count, Tsurf, cdf, enorm, fnorm, fmap, vert, faces, face_T;
     DevBuf density, horizon, vmom, vcount, ttime, end_state, steps, fate, hops;
     DevBuf vrec, vrec_i, count_nn, ctl;
     /* advanced */

@@ -49,6 +49,7 @@ enum { LANE_IDLE = 0, LANE_FLIGHT = 1, LANE_LANDED = 2, LANE_DONE = 3, LANE_ENDE
 enum { DSMC_VIRTUAL_RECORDS = 50 };   /* physical fits: P(vmin) = 50, as the reference's 50 records */
 enum { DSMC_VT_NT = 23, DSMC_VT_NV = 1000 };
 #define DSMC_FD_MARGIN 1e-12
+enum { DSMC_FMAP_K = 8 };             /* candidate faces per face-map cell */
 
 /* Everything a lane reads or writes; device pointers on the GPU. */
 struct DsmcEnv {
@@ -74,6 +75,8 @@ struct DsmcEnv {
     const int32_t *faces;     /* all levels, [sum*3] */
     const double *enorm;      /* all levels, [sum*9]: cross(a,b), cross(b,c), cross(c,a) */
     const double *fnorm;      /* finest level, [nfine*9]: the same normals, unit length */
+    const int32_t *fmap;      /* [6*G*G*DSMC_FMAP_K] candidate finest faces per cube-map cell, -1 = none */
+    int fmap_g;               /* G; fmap == nullptr: no map */
     int level_off[12];
     /* advanced inputs */
     const double *copy_atm;   /* scaled phase-1 density, [atm_total] */
@@ -755,18 +758,65 @@ DSMC_HD int locate_face(const DsmcEnv &E, double dx, double dy, double dz)
     return best;
 }
 
-/* locate_face with a guess: a direction at least 1e-12 |d| inside face `hint`
+/* Direction d at least eps inside finest face f (all three unit edge normals). */
+DSMC_HD bool face_holds(const DsmcEnv &E, int f, double dx, double dy, double dz, double eps)
+{
+    const double *u = E.fnorm + (size_t)f * 9;
+    const bool a = dx * u[0] + dy * u[1] + dz * u[2] >= eps;
+    const bool b = dx * u[3] + dy * u[4] + dz * u[5] >= eps;
+    const bool c = dx * u[6] + dy * u[7] + dz * u[8] >= eps;
+    return a & b & c;
+}
+
+/* Cube-map cell of direction d: the cube face its largest component points
+ * through (x+, x-, y+, y-, z+, z-; ties to the earlier axis), then a G x G grid
+ * over that face in gnomonic coordinates (the other two components over the
+ * largest: x-faces (y, z), y-faces (z, x), z-faces (x, y)), index
+ * (face * G + row) * G + column.  Float arithmetic: the cell only chooses which
+ * faces are tested, each exactly.  -1 for a zero or non-finite direction. */
+DSMC_HD int fmap_cell(int G, double dx, double dy, double dz)
+{
+    const float x = (float)dx, y = (float)dy, z = (float)dz;
+    const float ax = fabsf(x), ay = fabsf(y), az = fabsf(z);
+    int f;
+    float u, v, m;
+    if (ax >= ay && ax >= az) { f = x < 0.0f; m = ax; u = y; v = z; }
+    else if (ay >= az) { f = 2 + (y < 0.0f); m = ay; u = z; v = x; }
+    else { f = 4 + (z < 0.0f); m = az; u = x; v = y; }
+    if (!(m > 0.0f) || !(m <= 3.0e38f)) return -1;
+    const float half = 0.5f * (float)G, r = half / m;
+    const float a = fminf(fmaxf(u * r + half, 0.0f), (float)G - 1.0f);
+    const float b = fminf(fmaxf(v * r + half, 0.0f), (float)G - 1.0f);
+    return (f * G + (int)b) * G + (int)a;
+}
+
+/* locate_face with a guess: a direction at least 1e-12 |d| inside a finest face
  * (unit edge normals) is inside every ancestor of it by at least as much, and
  * outside every other face of each level, so the hierarchical search would
- * return `hint`; otherwise the search runs. */
+ * return that face.  Tested in turn: `hint`, then the candidates the face map
+ * holds for d's cube-map cell (four at a time, their normals loaded together);
+ * when none holds d that firmly, or |d| is zero or not finite, the search runs. */
+struct alignas(16) FmapQuad { int32_t f[4]; };
+
 DSMC_HD int locate_face_near(const DsmcEnv &E, int hint, double dx, double dy, double dz, double dlen)
 {
-    if (hint >= 0 && hint < E.nfine && E.fnorm) {
-        const double *u = E.fnorm + (size_t)hint * 9;
-        const double eps = 1e-12 * dlen;
-        if (dx * u[0] + dy * u[1] + dz * u[2] >= eps && dx * u[3] + dy * u[4] + dz * u[5] >= eps &&
-            dx * u[6] + dy * u[7] + dz * u[8] >= eps)
-            return hint;
+    if (!E.fnorm || !(dlen > 0.0 && dlen <= 1.0e308)) return locate_face(E, dx, dy, dz);
+    const double eps = 1e-12 * dlen;
+    if (hint >= 0 && hint < E.nfine && face_holds(E, hint, dx, dy, dz, eps)) return hint;
+    const int cell = E.fmap ? fmap_cell(E.fmap_g, dx, dy, dz) : -1;
+    if (cell >= 0) {
+        const FmapQuad *q = (const FmapQuad *)(E.fmap + (size_t)cell * DSMC_FMAP_K);
+#pragma unroll 1
+        for (int g = 0; g < DSMC_FMAP_K / 4; g++) {
+            const FmapQuad c = q[g];
+            if (c.f[0] < 0) break;
+            bool in[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) in[j] = face_holds(E, c.f[j] < 0 ? c.f[0] : c.f[j], dx, dy, dz, eps);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (in[j] && c.f[j] >= 0) return c.f[j];
+        }
     }
     return locate_face(E, dx, dy, dz);
 }

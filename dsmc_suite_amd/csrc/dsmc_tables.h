@@ -80,6 +80,7 @@ struct HostTables {
     std::vector<double> tq, hr, cs, vt_speeds;
     std::vector<int32_t> vt_count;
     std::vector<double> cdf, enorm, fnorm;
+    std::vector<int32_t> fmap;      /* build_face_map, where a surface is uploaded */
     int level_off[12];
     int levels = 0, nfine = 0, nface_all = 0;
 };
@@ -228,6 +229,68 @@ inline int build_env(const dsmc_params *p, const dsmc_surface *s, int advanced, 
         E->nfine = T->nfine;
     }
     return DSMC_OK;
+}
+
+/* Cube-map resolution of the face map: about 1.25 finest faces per cell (2-3
+ * candidates in a typical cell, at most 6 around a vertex), 8 <= G <= 128. */
+inline int face_map_grid(int nfine)
+{
+    int g = (int)lround(sqrt(nfine / 1.25));
+    return g < 8 ? 8 : (g > 128 ? 128 : g);
+}
+
+/* The face map of a physical surface (fmap_cell's cells, DSMC_FMAP_K candidates
+ * per cell, -1 padded): the finest faces holding the points of a lattice of
+ * 4 x 4 sub-cells per cell (corners and edges included, so neighbouring cells
+ * share their boundary points), most frequent first, lowest index on ties.  A
+ * face the lattice misses, or a cell with more than DSMC_FMAP_K faces, costs
+ * only the full search: locate_face_near tests every candidate exactly.
+ * `E` holds the surface scalars; enorm / fnorm are read from T. */
+inline void build_face_map(const DsmcEnv &E, const HostTables &T, int G, std::vector<int32_t> &out)
+{
+    const int S = 4, L = G * S + 1;
+    DsmcEnv H = E;
+    H.enorm = T.enorm.data();
+    H.fnorm = T.fnorm.data();
+    H.fmap = nullptr;
+    std::vector<int32_t> lat((size_t)L * L);
+    out.assign((size_t)6 * G * G * DSMC_FMAP_K, -1);
+    for (int cf = 0; cf < 6; cf++) {
+        const double sg = (cf & 1) ? -1.0 : 1.0;
+        const int axis = cf >> 1;
+        int hint = -1;
+        for (int j = 0; j < L; j++)
+            for (int i = 0; i < L; i++) {
+                const double u = -1.0 + 2.0 * i / (L - 1), v = -1.0 + 2.0 * j / (L - 1);
+                double d[3];
+                if (axis == 0) { d[0] = sg; d[1] = u; d[2] = v; }
+                else if (axis == 1) { d[0] = v; d[1] = sg; d[2] = u; }
+                else { d[0] = u; d[1] = v; d[2] = sg; }
+                const double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                hint = locate_face_near(H, hint, d[0], d[1], d[2], len);
+                lat[(size_t)j * L + i] = hint;
+            }
+        for (int cj = 0; cj < G; cj++)
+            for (int ci = 0; ci < G; ci++) {
+                int face[(S + 1) * (S + 1)], cnt[(S + 1) * (S + 1)], n = 0;
+                for (int j = cj * S; j <= (cj + 1) * S; j++)
+                    for (int i = ci * S; i <= (ci + 1) * S; i++) {
+                        const int f = lat[(size_t)j * L + i];
+                        int k = 0;
+                        while (k < n && face[k] != f) k++;
+                        if (k == n) { face[n] = f; cnt[n++] = 0; }
+                        cnt[k]++;
+                    }
+                int32_t *o = &out[((size_t)(cf * G + cj) * G + ci) * DSMC_FMAP_K];
+                for (int k = 0; k < DSMC_FMAP_K && k < n; k++) {
+                    int b = -1;
+                    for (int q = 0; q < n; q++)
+                        if (cnt[q] > 0 && (b < 0 || cnt[q] > cnt[b] || (cnt[q] == cnt[b] && face[q] < face[b]))) b = q;
+                    o[k] = face[b];
+                    cnt[b] = 0;
+                }
+            }
+    }
 }
 
 }  // namespace dsmc

@@ -111,12 +111,16 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
         UP(rd->cdf, T.cdf);
         UP(rd->enorm, T.enorm);
         UP(rd->fnorm, T.fnorm);
+        E.fmap_g = face_map_grid(T.nfine);
+        build_face_map(E, T, E.fmap_g, T.fmap);
+        UP(rd->fmap, T.fmap);
         if ((rc = rd->vert.upload(s->vert, sizeof(double) * 3 * s->nvert, st))) return rc;
         if ((rc = rd->faces.upload(s->faces, sizeof(int32_t) * 3 * T.nface_all, st))) return rc;
         if ((rc = rd->face_T.upload(s->face_T, sizeof(double) * T.nfine, st))) return rc;
         E.cdf = (const double *)rd->cdf.ptr;
         E.enorm = (const double *)rd->enorm.ptr;
         E.fnorm = (const double *)rd->fnorm.ptr;
+        E.fmap = (const int32_t *)rd->fmap.ptr;
         E.vert = (const double *)rd->vert.ptr;
         E.faces = (const int32_t *)rd->faces.ptr;
         E.face_T = (const double *)rd->face_T.ptr;

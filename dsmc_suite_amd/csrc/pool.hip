@@ -222,6 +222,9 @@ int pool_create(Pool *P, const PoolClass &c, int max_runs, int64_t max_slots, in
     if ((rc = P->geo_faces.alloc(POOL_MAX_GEO * P->nface_all * 3 * 4))) return rc;
     if ((rc = P->geo_enorm.alloc(POOL_MAX_GEO * P->nface_all * 9 * 8))) return rc;
     if ((rc = P->geo_fnorm.alloc(POOL_MAX_GEO * P->nfine * 9 * 8))) return rc;
+    P->fmap_g = face_map_grid((int)P->nfine);
+    const size_t fmap_n = (size_t)6 * P->fmap_g * P->fmap_g * DSMC_FMAP_K;
+    if ((rc = P->geo_fmap.alloc(POOL_MAX_GEO * fmap_n * 4))) return rc;
     if ((rc = P->cdf.alloc(mr * P->nfine * 8)) || (rc = P->face_T.alloc(mr * P->nfine * 8))) return rc;
     if ((rc = P->vcount.alloc(mr * P->cells * 8)) || (rc = P->vmom.alloc(mr * P->cells * 24))) return rc;
     if ((rc = P->horizon.alloc(mr * c.ftd * c.ffd * 8)) || (rc = P->counters.zeros(mr * 64, st))) return rc;
@@ -424,6 +427,8 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out, int hist, 
     int32_t *faces_d = (int32_t *)P->geo_faces.ptr + (size_t)geo * P->nface_all * 3;
     double *enorm_d = (double *)P->geo_enorm.ptr + (size_t)geo * P->nface_all * 9;
     double *fnorm_d = (double *)P->geo_fnorm.ptr + (size_t)geo * P->nfine * 9;
+    const size_t fmap_n = (size_t)6 * P->fmap_g * P->fmap_g * DSMC_FMAP_K;
+    int32_t *fmap_d = (int32_t *)P->geo_fmap.ptr + (size_t)geo * fmap_n;
     if (new_geo) {
         /* a slot no queued work reads (its last user has finished); synchronous
          * copies from the caller's pageable arrays, once per geometry */
@@ -431,6 +436,9 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out, int hist, 
         HIP_OK(hipMemcpy(faces_d, pr.faces.data(), P->nface_all * 12, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(enorm_d, T.enorm.data(), P->nface_all * 72, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(fnorm_d, T.fnorm.data(), P->nfine * 72, hipMemcpyHostToDevice));
+        std::vector<int32_t> fm;
+        build_face_map(pr.E, T, P->fmap_g, fm);
+        HIP_OK(hipMemcpy(fmap_d, fm.data(), fmap_n * 4, hipMemcpyHostToDevice));
         P->geos[geo].key = key;
         P->geos[geo].valid = 1;
     }
@@ -456,6 +464,8 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out, int hist, 
     E.faces = faces_d;
     E.enorm = enorm_d;
     E.fnorm = fnorm_d;
+    E.fmap = fmap_d;
+    E.fmap_g = P->fmap_g;
     E.density = nullptr;
     E.density_from_count = 1;
     E.vcount = (unsigned long long *)P->vcount.ptr + (size_t)g * cells;

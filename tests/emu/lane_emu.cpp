@@ -26,6 +26,15 @@ static void run_one(DsmcLane &L, const DsmcEnv &E, int64_t n, DsmcCounts &cnt)
  * `budget` steps, rec_store_flight); service (rec_load_all, lane_service,
  * rec_store_all).  INLINE (physical): start -> record; launches of
  * rec_load_phys, steps with the service inline, rec_store_phys on a pause. */
+/* The face map, as the library attaches it where a physical surface is uploaded. */
+static void attach_face_map(DsmcEnv &E, dsmc::HostTables &T)
+{
+    if (E.mode != DSMC_MODE_PHYSICAL || E.nfine <= 0) return;
+    E.fmap_g = dsmc::face_map_grid(E.nfine);
+    dsmc::build_face_map(E, T, E.fmap_g, T.fmap);
+    E.fmap = T.fmap.data();
+}
+
 static void garbage(DsmcLane &L) { memset(&L, 0xA5, sizeof L); }
 
 template <int MODE, int ADV>
@@ -156,6 +165,7 @@ extern "C" int emu_ballistic_path(const dsmc_params *p, const dsmc_surface *s, d
     E.cdf = T.cdf.data(); E.face_T = s->face_T; E.vert = s->vert; E.faces = s->faces;
     E.enorm = T.enorm.data();
     E.fnorm = T.fnorm.data();
+    attach_face_map(E, T);
     E.density = den.data(); E.horizon = hor.data();
     E.vmom = r->vmom;
     if (r->vmom) memset(r->vmom, 0, cells * 3 * sizeof(double));
@@ -218,6 +228,7 @@ extern "C" int emu_physical_coll(const dsmc_params *p, const dsmc_surface *s, co
     E.tq = T.tq.data(); E.hr = T.hr.data(); E.cs = T.cs.data();
     E.cdf = T.cdf.data(); E.face_T = s->face_T; E.vert = s->vert; E.faces = s->faces;
     E.enorm = T.enorm.data(); E.fnorm = T.fnorm.data();
+    attach_face_map(E, T);
     E.density = den.data(); E.horizon = hor.data();
     E.vmom = r->vmom;
     if (r->vmom) memset(r->vmom, 0, cells * 3 * sizeof(double));
@@ -239,5 +250,27 @@ extern "C" int emu_physical_coll(const dsmc_params *p, const dsmc_surface *s, co
     r->n_escaped = cnt.esc; r->n_frozen = cnt.frozen; r->n_photolysed = cnt.photo;
     r->n_depleted = cnt.depleted; r->n_capped = cnt.capped; r->n_collisions = cnt.coll;
     r->total_steps = cnt.steps;
+    return 0;
+}
+
+/* Face location of n directions: out_near = locate_face_near (hint, then the
+ * face map, then the search), out_full = the hierarchical search alone. */
+extern "C" int emu_locate(const dsmc_params *p, const dsmc_surface *s, int64_t n, const double *dirs,
+                          const int32_t *hints, int32_t *out_near, int32_t *out_full)
+{
+    if (p->mode != DSMC_MODE_PHYSICAL) return -1;
+    DsmcEnv E;
+    dsmc::HostTables T;
+    int rc = dsmc::build_env(p, s, 0, &E, &T);
+    if (rc) return rc;
+    E.enorm = T.enorm.data();
+    E.fnorm = T.fnorm.data();
+    attach_face_map(E, T);
+    for (int64_t i = 0; i < n; i++) {
+        const double *d = dirs + 3 * i;
+        const double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        out_near[i] = locate_face_near(E, hints ? hints[i] : -1, d[0], d[1], d[2], len);
+        out_full[i] = locate_face(E, d[0], d[1], d[2]);
+    }
     return 0;
 }

@@ -120,6 +120,55 @@ def test_icosphere(lib, levels):
     assert np.all(fw >= 0) and np.all(np.diff(flux[np.argsort(fT)]) >= -1e-12 * flux.max())
 
 
+@pytest.mark.parametrize("levels", [0, 3, 4, 5])
+def test_face_map_locates_as_the_full_search(lib, emu, levels):
+    """locate_face_near (hint, then the cube-map candidates, then the search) gives
+    the hierarchical search's face for random directions and for the hard ones:
+    vertices, edge midpoints, points 1e-13..1e-9 off every kind of edge, cube
+    edges and face-map cell boundaries, zero and non-finite directions."""
+    vert, faces, fT, fw = build_icosphere(lib, levels)
+    nfine = 20 * 4 ** levels
+    fine = faces[-nfine:]
+    rng = np.random.default_rng(levels)
+    R = 470e3
+    d = [rng.normal(size=(20000, 3)) * R, vert * R]
+    a, b, c = vert[fine[:, 0]], vert[fine[:, 1]], vert[fine[:, 2]]
+    d += [(a + b + c) * R, (a + b) * R / 2]
+    for p0, p1 in ((a, b), (b, c), (c, a)):
+        t = rng.uniform(size=(nfine, 1))
+        on = p0 + t * (p1 - p0)
+        nrm = np.cross(p0, p1)
+        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+        for delta in (1e-13, 1e-12, 1e-11, 1e-9):
+            d += [(on + delta * nrm) * R, (on - delta * nrm) * R]
+    G = int(min(128, max(8, round((nfine / 1.25) ** 0.5))))
+    k = np.arange(G + 1) * 2.0 / G - 1.0
+    u, v = np.meshgrid(k, rng.uniform(-1, 1, 7))
+    for col in range(3):                                    # cell boundaries on each cube face
+        for s in (1.0, -1.0):
+            q = np.zeros((u.size, 3))
+            q[:, col] = s
+            q[:, (col + 1) % 3] = u.ravel()
+            q[:, (col + 2) % 3] = v.ravel()
+            d += [q * R, q[:, [0, 2, 1]] * R]
+    w = rng.uniform(-1, 1, 500)
+    d += [np.stack([np.ones_like(w), np.ones_like(w), w], 1), np.stack([w, -np.ones_like(w), np.ones_like(w)], 1)]
+    d += [np.array([[0.0, 0.0, 0.0], [np.inf, 1.0, 0.0], [1e300, 1e300, 1.0], [1e-300, 0.0, 0.0]])]
+    dirs = np.ascontiguousarray(np.concatenate(d))
+    n = len(dirs)
+    hints = rng.integers(-1, nfine, n).astype(np.int32)
+    surf = SurfaceArrays(levels=levels, vert=vert.ravel(), faces=faces.ravel(), face_T=fT, face_w=fw)
+    p = capi.preset(lib, PRESET_CERES_PHYSICAL, 36, 72, n=1, seed=1)
+    near, full = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    emu.emu_locate.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + [C.c_void_p] * 4
+    for h in (None, hints):
+        assert emu.emu_locate(C.byref(p), C.byref(surf.surface), n, dirs.ctypes.data,
+                              None if h is None else h.ctypes.data, near.ctypes.data, full.ctypes.data) == 0
+        bad = np.nonzero(near != full)[0]
+        assert len(bad) == 0, (len(bad), dirs[bad[:5]], near[bad[:5]], full[bad[:5]])
+    assert len(np.unique(full)) == nfine
+
+
 @pytest.mark.parametrize("tname", ["T_thermal", "T_polar"])
 def test_lane_state_machine_matches_oracle_compat(emu, tname):
     dev = oracle(devmath=True)
