@@ -368,7 +368,9 @@ DSMC_HD void lane_flush(DsmcLane &L, const EnvT &E)
     L.acc_cell = -1;
 }
 
-template <class Ops, class EnvT>
+/* WEIGHT: the visit's weight sum is kept (compat runs); physical particles all
+ * weigh 1 and their density is the visit count, so they skip it. */
+template <class Ops, bool WEIGHT = true, class EnvT>
 DSMC_HD void lane_deposit(DsmcLane &L, const EnvT &E, int cell)
 {
     if (cell != L.acc_cell) {
@@ -376,7 +378,7 @@ DSMC_HD void lane_deposit(DsmcLane &L, const EnvT &E, int cell)
         L.acc_cell = cell;
         L.acc_w = 0.0; L.acx = 0.0; L.acy = 0.0; L.acz = 0.0; L.acc_n = 0;
     }
-    L.acc_w += L.W;
+    if (WEIGHT) L.acc_w += L.W;
     L.acx += L.vx; L.acy += L.vy; L.acz += L.vz;
     L.acc_n++;
 }
@@ -1091,7 +1093,7 @@ DSMC_HD void lane_step(DsmcLane &L, const EnvT &E, const DsmcCellTabs &TB)
             if constexpr (ADV != 0) {
                 if (cell != L.ncell) { L.ncur = E.nsig[cell]; L.ncell = cell; }
             }
-            lane_deposit<Ops>(L, E, cell);
+            lane_deposit<Ops, false>(L, E, cell);
         } else if constexpr (ADV != 0) {
             L.ncur = 0.0;
             L.ncell = -1;

@@ -118,7 +118,7 @@ struct DsmcOpsLds {
             if (t == -1) t = cell;
         }
         if (t != cell) { grid_deposit(E, cell, w, vx, vy, vz, n); return; }
-        atomicAdd(&dep_w[s], w);
+        if (!E.density_from_count) atomicAdd(&dep_w[s], w);
         if (E.vmom) { atomicAdd(&dep_vx[s], vx); atomicAdd(&dep_vy[s], vy); atomicAdd(&dep_vz[s], vz); }
         if (E.vcount) atomicAdd(&dep_n[s], n);
     }

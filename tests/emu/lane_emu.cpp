@@ -169,7 +169,9 @@ extern "C" int emu_ballistic_path(const dsmc_params *p, const dsmc_surface *s, d
     E.density = den.data(); E.horizon = hor.data();
     E.vmom = r->vmom;
     if (r->vmom) memset(r->vmom, 0, cells * 3 * sizeof(double));
-    E.vcount = r->vcount ? vcount.data() : nullptr;
+    /* physical runs: density = visit count, as the library computes it */
+    E.density_from_count = p->mode == DSMC_MODE_PHYSICAL;
+    E.vcount = r->vcount || E.density_from_count ? vcount.data() : nullptr;
     E.ttime = p->mode == DSMC_MODE_COMPAT ? r->ttime : nullptr;
     E.ttime_rows = 5;
     if (E.ttime) memset(E.ttime, 0, sizeof(double) * 5 * p->nparticles);
@@ -194,7 +196,8 @@ extern "C" int emu_ballistic_path(const dsmc_params *p, const dsmc_surface *s, d
         else if (p->mode == DSMC_MODE_COMPAT) run_one<0, 0>(L, E, n, cnt);
         else run_one<1, 0>(L, E, n, cnt);
     }
-    if (r->density) memcpy(r->density, den.data(), cells * sizeof(double));
+    if (r->density && E.density_from_count) for (size_t i = 0; i < cells; i++) r->density[i] = (double)vcount[i];
+    else if (r->density) memcpy(r->density, den.data(), cells * sizeof(double));
     if (r->horizon) memcpy(r->horizon, hor.data(), hor.size() * sizeof(double));
     if (r->vcount) for (size_t i = 0; i < cells; i++) r->vcount[i] = (int64_t)vcount[i];
     if (E.vhist) for (size_t i = 0; i < vh.size(); i++) r->vhist[i] = (int64_t)vh[i];
