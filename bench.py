@@ -43,7 +43,7 @@ FP64_PEAK_TFLOPS = 78.6
 # (per dispatch, averaged over the run's launches), each scaled by what the
 # counter reads for a plain 1 GiB device-to-device copy (FETCH_SIZE x 2.00,
 # WRITE_SIZE x 1.00).  Only that configuration reports it; others give null.
-PMC_TRAFFIC_1E7 = 310_172_860 + 7_782_874_893      # read + written, bytes per launch
+PMC_TRAFFIC_1E7 = 153_930_711 + 3_356_706_875      # read + written, bytes per launch
 
 
 def icosphere(lib, levels, lon=0.0, mass=2.99e-26, kB=1.380649e-23):

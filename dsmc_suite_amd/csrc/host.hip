@@ -254,10 +254,16 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
     if ((int64_t)flight_grid > blocks) flight_grid = (int)blocks;
     int service_grid = start_grid;
     /* per-launch step budget: long hops are split over rounds so the run's tail is
-     * set by its longest particle, not by the sum of each round's longest hop */
+     * set by its longest particle, not by the sum of each round's longest hop.
+     * The default shape of the split kernels runs its first two rounds with the
+     * full budget and later ones with at most 1024 steps: by then few particles
+     * fly, mostly multi-hop ones whose next hop waits for the round's end (Ceres
+     * 1e7: 203.6 against 210.5 ms per run with 4096 throughout, 258-282 ms with
+     * 8192-20000 in the later rounds) */
     const int64_t sh_budget = sh.step_budget;
+    const int64_t late_budget = !shape && !inline_svc && sh_budget > 1024 ? 1024 : sh_budget;
     std::vector<hipEvent_t> ev;
-    int p = 0, chunk = 4;
+    int p = 0, chunk = 4, issued = 0;
     for (;;) {
         for (int r = 0; r < chunk; r++) {
             if (timed) {
@@ -273,12 +279,13 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
                                         sh.block, sh.relaunch_batch, sh_budget, st));
                 if (timed) HIP_OK(hipEventRecord(ev.back(), st));
             } else {
-                HIP_OK(dsmc_launch_flight(&rd->E, rec, Q, p, flight_grid, sh.block, sh.relaunch_batch, sh_budget,
-                                          st));
+                HIP_OK(dsmc_launch_flight(&rd->E, rec, Q, p, flight_grid, sh.block, sh.relaunch_batch,
+                                          issued < 2 ? sh_budget : late_budget, st));
                 if (timed) HIP_OK(hipEventRecord(ev.back(), st));
                 HIP_OK(dsmc_launch_service(&rd->E, rec, Q, p, service_grid, st));
             }
             p ^= 1;
+            issued++;
         }
         HIP_OK(hipMemcpyAsync(rd->host_ctl, Q.ctl, sizeof(unsigned int) * 16, hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
@@ -318,8 +325,9 @@ dsmc_launch_shape default_shape(const DsmcEnv &E, int short_hops)
         s.service_inline = 1;
     } else {
         /* long hops (Ceres): the lean flight kernel.  1e7 particles: 5 blocks per
-         * CU and a 4096-step budget 1.22e11 particle-steps/s (4 blocks, 1024
-         * steps: 1.13e11); runs of 1e4-2e6 particles do best with 1024 steps */
+         * CU and a 4096-step budget in the first two rounds (1024 after them)
+         * 1.31e11 particle-steps/s (4 blocks, 1024 steps throughout: 1.13e11);
+         * runs of 1e4-2e6 particles do best with 1024 steps throughout */
         s.blocks_per_cu = 5;
         s.relaunch_batch = 16;
         s.step_budget = E.nparticles >= 4000000 ? 4096 : 1024;
