@@ -362,6 +362,12 @@ int read_v_record_dat(const char *filename, double **v_record, int **td_list,
 double *read_flux_dat(const char *filename, int *nface, int *tt);
 int write_flux_dat(const char *filename, const double *F, int nface, int tt);
 int write_horizon_dat(const char *filename, const double *horizon, int ftd, int ffd);
+/* Host threads of the functions above (they format and parse chunks of a file
+ * concurrently; the bytes written and values read do not depend on it): n > 0
+ * sets it, 0 restores the default (the machine's cores, at most 16).  Returns
+ * the previous setting.  The drop-in drivers use their own nthreads.  No
+ * counterpart in the reference, whose file functions are serial. */
+int dsmc_io_threads(int n);
 
 /* ------------------------------------------------------------------ misc */
 const char *dsmc_version(void);
