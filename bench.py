@@ -31,7 +31,7 @@ sys.path.insert(0, str(ROOT))
 
 from dsmc_suite_amd import capi  # noqa: E402
 from dsmc_suite_amd import workloads as W  # noqa: E402
-from dsmc_suite_amd.capi import PRESET_CERES_PHYSICAL, Buffers, SurfaceArrays, ptr  # noqa: E402
+from dsmc_suite_amd.capi import PRESET_CERES_PHYSICAL, PRESET_COMPAT_BALLISTIC, Buffers, SurfaceArrays, ptr  # noqa: E402
 
 METRIC = "particle-steps/sec FP64, plain vs batched/served, 1 MI355X; p50/p99 run latency"
 # Algorithmic FP64 operations of one particle-step of the push (Verlet 51 +
@@ -313,6 +313,65 @@ def served_run(lib, args, shape, rate: float, seconds: float) -> dict:
             "slots": int(stats.slots)}
 
 
+def dropin_run(lib, args) -> dict:
+    """The work behind one drop-in run_ballistic call at the reference's own size
+    (compat ballistic, 1e4 particles, 36x72 surface field, 100 uniform shells):
+    the device run (staging, particles, results back) and its four text files,
+    written and read back (as run_advanced reads them) with the default host
+    threads and with one."""
+    import tempfile
+    ftd, ffd, n = 36, 72, 10_000
+    lat = (np.arange(ftd) + 0.5) / ftd * np.pi - np.pi / 2
+    lon = (np.arange(ffd) + 0.5) / ffd * 2 * np.pi
+    T = 80.0 + 160.0 * np.maximum(np.outer(np.cos(lat), np.sin(lon)), 0.0) ** 0.25
+    T = np.ascontiguousarray(T.ravel())
+    p = capi.preset(lib, PRESET_COMPAT_BALLISTIC, ftd, ffd, n=n, seed=args.seed)
+    surf = SurfaceArrays(T, ftd, ffd)
+    b = Buffers(p, 5, per_particle=True, vrec=True)
+    dev = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        if lib.dsmc_run_ballistic(C.byref(p), C.byref(surf.surface), C.byref(b.result)):
+            raise RuntimeError("dsmc_run_ballistic failed")
+        dev.append((time.perf_counter() - t0) * 1e3)
+    nrec = int(b.result.vrec_n)
+    dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+    lib.read_density_dat.restype = dp
+    lib.read_v_record_dat.argtypes = [C.c_char_p, C.POINTER(dp), C.POINTER(ip), C.POINTER(ip), C.POINTER(ip), ip]
+    out = {"particles": n, "particle_steps": int(b.result.total_steps), "v_records": nrec,
+           "device_ms": round(float(np.median(dev)), 2)}
+    with tempfile.TemporaryDirectory() as d:
+        for label, threads in (("", 0), ("_1thread", 1)):
+            lib.dsmc_io_threads(threads)
+            f = {k: os.path.join(d, f"r{label}_{k}.dat").encode() for k in ("den", "horizon", "v_record", "ttime")}
+            w, r = [], []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                rc = (lib.write_density_dat(f["den"], b.density.ctypes.data_as(dp), ftd, ffd, p.hd) |
+                      lib.write_horizon_dat(f["horizon"], b.horizon.ctypes.data_as(dp), ftd, ffd) |
+                      lib.write_v_record_dat(f["v_record"], b.vrec.ctypes.data_as(dp), b.vrec_td.ctypes.data_as(ip),
+                                             b.vrec_fd.ctypes.data_as(ip), b.vrec_ht.ctypes.data_as(ip), nrec) |
+                      lib.write_ttime_dat(f["ttime"], b.ttime.ctypes.data_as(dp), 5, n))
+                w.append((time.perf_counter() - t0) * 1e3)
+                if rc:
+                    raise RuntimeError("writing the .dat files failed")
+                a1, a2, a3 = C.c_int(), C.c_int(), C.c_int()
+                v, vt, vf, vh, nr = dp(), ip(), ip(), ip(), C.c_int()
+                t0 = time.perf_counter()
+                den = lib.read_density_dat(f["den"], C.byref(a1), C.byref(a2), C.byref(a3), 1)
+                ok = lib.read_v_record_dat(f["v_record"], C.byref(v), C.byref(vt), C.byref(vf), C.byref(vh),
+                                           C.byref(nr)) == 0
+                r.append((time.perf_counter() - t0) * 1e3)
+                if not den or not ok or nr.value != nrec:
+                    raise RuntimeError("reading the .dat files back failed")
+            out[f"write_ms{label}"] = round(min(w), 2)
+            out[f"read_ms{label}"] = round(min(r), 2)
+        out["text_bytes"] = sum(os.path.getsize(x) for x in f.values())
+    lib.dsmc_io_threads(0)
+    out["io_threads"] = min(16, os.cpu_count() or 1)
+    return out
+
+
 def copy_calibration(mib: int) -> None:
     """A plain device-to-device copy of known size: what the HBM byte counters
     read for it (per dispatch) against its 2 x size algorithmic bytes gives the
@@ -357,6 +416,7 @@ def main():
     ap.add_argument("--serve-budget-gb", type=int, default=128)
     ap.add_argument("--serve-collisional", type=float, default=0.3, help="fraction of collisional requests")
     ap.add_argument("--no-coll", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in run_ballistic files leg")
     ap.add_argument("--coll-particles", type=int, default=2_000_000)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only copy a buffer of this many MiB device to device 5 times (HBM counter calibration)")
@@ -397,6 +457,9 @@ def main():
         line["config"][f"plain_{args.also_particles:.0e}"] = {
             "value": other["value"], "ms_per_run": other["ms_per_run"],
             "flight_kernel_ms": other["flight_kernel_ms"], "rounds": other["rounds"]}
+    if not args.no_dropin:
+        progress("drop-in run_ballistic work at 1e4 particles (device run + .dat files)")
+        line["dropin_files"] = dropin_run(lib, args)
     tuned = None
     if not args.no_batch:
         progress("batched: 64 runs (autotune, launches, plain sequential)")

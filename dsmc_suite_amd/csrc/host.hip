@@ -12,6 +12,7 @@
 #include <string.h>
 #include <math.h>
 #include <time.h>
+#include <limits.h>
 #include <vector>
 #include <mutex>
 
@@ -224,15 +225,18 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
     if (rc) return rc;
     dsmc_launch_shape sh = shape ? *shape : default_shape(rd->E, rd->short_hops);
     if (!shape_ok(sh)) return DSMC_ERR_ARG;
-    /* physical collisional runs always service inline (collisions are service events) */
-    const bool inline_svc = rd->E.mode == DSMC_MODE_PHYSICAL && (sh.service_inline || rd->E.advanced);
+    /* physical collisional runs always service inline (collisions are service
+     * events); compat advanced runs never (no kernel of that kind) */
+    const bool compat_adv = rd->E.mode == DSMC_MODE_COMPAT && rd->E.advanced;
+    const bool inline_svc = (sh.service_inline && !compat_adv) || (rd->E.mode == DSMC_MODE_PHYSICAL && rd->E.advanced);
     if (!shape) {
         /* the default asks for up to blocks_per_cu resident blocks; fewer if the
          * kernel's registers or LDS do not allow them */
         int occ = dsmc_flight_occupancy(&rd->E, inline_svc ? 1 : 0, sh.block);
         if (occ > 0 && occ < sh.blocks_per_cu) sh.blocks_per_cu = occ;
-        /* collisional runs (inline by necessity): 1024-step launches */
-        if (rd->E.advanced && inline_svc && sh.step_budget > 1024) sh.step_budget = 1024;
+        /* physical collisional runs (inline by necessity): 1024-step launches */
+        if (rd->E.mode == DSMC_MODE_PHYSICAL && rd->E.advanced && inline_svc && sh.step_budget > 1024)
+            sh.step_budget = 1024;
     }
     const int64_t n = rd->E.nparticles;
     DsmcQueues Q;
@@ -310,7 +314,19 @@ dsmc_launch_shape default_shape(const DsmcEnv &E, int short_hops)
 {
     dsmc_launch_shape s;
     s.block = 256;
-    if (E.mode == 0) {
+    if (E.mode == 0 && !E.advanced) {
+        /* compat ballistic particles make hundreds of hops, many of one step:
+         * with the service in its own kernel every hop is a round of launches
+         * and a run lasts as many rounds as its most-hopping particle has hops
+         * (11900 at 1e4 particles, ~0.14 ms each).  With the service inline and
+         * no step budget a run is one launch: 1e4 particles 254 ms against
+         * 1634 ms, 1e6 708 ms against 5320 ms */
+        s.blocks_per_cu = 4;
+        s.relaunch_batch = 48;
+        s.step_budget = INT_MAX;
+        s.service_inline = 1;
+    } else if (E.mode == 0) {
+        /* compat advanced (few, long hops at dt 0.5 s): the split kernels */
         s.blocks_per_cu = 4;
         s.relaunch_batch = 8;
         s.step_budget = 4096;

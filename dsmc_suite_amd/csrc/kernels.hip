@@ -387,6 +387,14 @@ __device__ __attribute__((noinline)) void phys_service(DsmcLane &T, const DsmcEn
     lane_service<Ops, 1, ADV>(T, E, cnt);
 }
 
+/* The compat landing service (relaunch with the frozen scan, polar deposit,
+ * outputs of a finished particle), out of line for the same reason. */
+__device__ __attribute__((noinline)) void compat_service(DsmcLane &T, const DsmcEnv *Ep, DsmcCounts &cnt)
+{
+    const DsmcEnv E = *Ep;
+    lane_service<DsmcOpsLds, 0, 0>(T, E, cnt);
+}
+
 /* A collision (physical collisional runs), out of line with only what it reads
  * and writes passed in registers: velocity, random-stream counter, cell. */
 struct CollideOut {
@@ -437,11 +445,12 @@ __device__ __forceinline__ void lane_collide_lean(DsmcLane &L, const DsmcEnv *Ep
 /* Pools: up to 168 VGPRs (3 waves per SIMD), so that the lanes' per-run step
  * constants do not spill inside the flight loop (a reload waits behind the
  * wavefront's outstanding deposit atomics); single runs: 128 (4 waves). */
-template <bool POOL, int ADV>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(POOL ? 3 : 4)))
-phys_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcPoolDev P, DsmcRec *rec, DsmcQueues Q, int p, int svc_batch,
-            int64_t step_budget)
+template <bool POOL, int MODE, int ADV>
+__device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const DsmcEnv *Ed, const DsmcPoolDev &P,
+                                                    DsmcRec *rec, const DsmcQueues &Q, int p, int svc_batch,
+                                                    int64_t step_budget)
 {
+    static_assert(MODE == 1 || (!POOL && !ADV), "pools hold physical runs; compat inline is ballistic");
     extern __shared__ double lds_tabs[];
     const int ftd = POOL ? P.ftd : E0.ftd, ffd = POOL ? P.ffd : E0.ffd, hd = POOL ? P.hd : E0.hd;
     const int ntq = ftd + 1, ncs = 2 * (ffd + 1), nhr = POOL ? P.nhr : hd + 2;
@@ -485,7 +494,8 @@ phys_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcPoolDev P, DsmcRec *rec, DsmcQueu
                         S = P.steps[run];
                         TB.hr = lds_hr + S.hr_off;
                     }
-                    rec_load_phys(L, rec[slot]);
+                    if (MODE == 0) rec_load_all<0>(L, rec[slot]);
+                    else rec_load_phys(L, rec[slot]);
                     budget_end = step_budget > 0 ? L.steps + step_budget : INT64_MAX;
                 }
             }
@@ -502,40 +512,47 @@ phys_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcPoolDev P, DsmcRec *rec, DsmcQueu
 #pragma unroll
             for (int u = 0; u < 2; u++) {
                 if (L.phase == LANE_FLIGHT) {
-                    if (POOL) lane_step<DsmcOpsDev, 1, ADV>(L, S, TB);
-                    else lane_step<DsmcOpsLds, 1, ADV>(L, E0, TB);
+                    if constexpr (POOL) lane_step<DsmcOpsDev, 1, ADV>(L, S, TB);
+                    else lane_step<DsmcOpsLds, MODE, ADV>(L, E0, TB);
                     if (L.phase == LANE_FLIGHT && L.steps >= budget_end) L.phase = LANE_IDLE + 8;
                 }
             }
             /* collisions in a dense gas come every few steps: once COLLIDE_BATCH
              * lanes wait for one they take the lean path here, without ending
              * the flight loop; fewer wait for the service below */
-            if (ADV && __popcll(__ballot(L.phase == LANE_COLLIDE)) >= COLLIDE_BATCH && L.phase == LANE_COLLIDE)
+            if (MODE == 1 && ADV && __popcll(__ballot(L.phase == LANE_COLLIDE)) >= COLLIDE_BATCH &&
+                L.phase == LANE_COLLIDE)
                 lane_collide_lean(L, POOL ? P.envs + run : Ed);
             uint64_t ended = __ballot(L.phase != LANE_FLIGHT && L.phase != LANE_IDLE);
             uint64_t waiting = ended | __ballot(L.phase == LANE_IDLE);
             if ((ended && __popcll(waiting) >= svc_batch) || waiting == __ballot(true)) break;
         }
         /* the remaining collisions: the lean path (the deposit visit stays open) */
-        if (ADV && __ballot(L.phase == LANE_COLLIDE) && L.phase == LANE_COLLIDE) lane_collide_lean(L, POOL ? P.envs + run : Ed);
+        if (MODE == 1 && ADV && __ballot(L.phase == LANE_COLLIDE) && L.phase == LANE_COLLIDE)
+            lane_collide_lean(L, POOL ? P.envs + run : Ed);
         /* landings and ended particles: service inline */
         DsmcCounts cnt = {0, 0, 0, 0, 0, 0, 0, 0};
         const bool svc = L.phase == LANE_LANDED || L.phase == LANE_ENDED || L.phase == LANE_COLLIDE;
         if (svc) {
             DsmcLane T = L;
-            if (POOL) phys_service<DsmcOpsDev, ADV>(T, P.envs + run, cnt, TB);
+            if constexpr (MODE == 0) compat_service(T, Ed, cnt);
+            else if constexpr (POOL) phys_service<DsmcOpsDev, ADV>(T, P.envs + run, cnt, TB);
             else phys_service<DsmcOpsLds, ADV>(T, Ed, cnt, TB);
             L = T;
         }
         const bool fin = svc && L.phase == LANE_DONE;
         if (__ballot(fin)) counts_flush_wave(cnt, fin ? (POOL ? P.envs[run].counters : E0.counters) : nullptr, fin);
         if (fin) L.phase = LANE_IDLE;
-        /* budget-paused particles go to the next launch, their open deposit
-         * visit with them (a launch boundary does not split a visit) */
+        /* budget-paused particles go to the next launch; a physical one takes its
+         * open deposit visit with it (a launch boundary does not split a visit),
+         * a compat one flushes it, as the split flight kernel does */
         bool paused = L.phase == LANE_IDLE + 8;
         if (paused) {
             L.phase = LANE_FLIGHT;
-            rec_store_phys(L, rec[slot]);
+            if (MODE == 0) {
+                lane_flush<DsmcOpsLds>(L, E0);
+                rec_store_all<0>(L, rec[slot], 0);
+            } else rec_store_phys(L, rec[slot]);
         }
         wave_append(paused, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
         if (paused) L.phase = LANE_IDLE;
@@ -544,6 +561,26 @@ phys_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcPoolDev P, DsmcRec *rec, DsmcQueu
         __syncthreads();
         dep_merge(E0);
     }
+}
+
+template <bool POOL, int ADV>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(POOL ? 3 : 4)))
+phys_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcPoolDev P, DsmcRec *rec, DsmcQueues Q, int p, int svc_batch,
+            int64_t step_budget)
+{
+    inline_service_loop<POOL, 1, ADV>(E0, Ed, P, rec, Q, p, svc_batch, step_budget);
+}
+
+/* Compat ballistic particles (the reference's run_ballistic) with the landing
+ * service inline: each of a particle's hops (hundreds; many of one step) would
+ * otherwise cost a round of launches, so a small run's time would be its
+ * most-hopping particle's hop count times a round's latency.  (Compat advanced
+ * runs keep the split kernels: inline they were 10% slower.) */
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+compat_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcRec *rec, DsmcQueues Q, int p, int svc_batch, int64_t step_budget)
+{
+    DsmcPoolDev none = {};
+    inline_service_loop<false, 0, 0>(E0, Ed, none, rec, Q, p, svc_batch, step_budget);
 }
 
 /* ------------------------------------------------------------------ service */
@@ -686,6 +723,8 @@ extern "C" int dsmc_flight_occupancy(const DsmcEnv *E, int inline_svc, int block
     if (inline_svc && mode == 1) {
         if (adv) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, phys_kernel<false, 1>, block, lds);
         else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, phys_kernel<false, 0>, block, lds);
+    } else if (inline_svc && !adv) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, compat_kernel, block, lds);
     } else if (mode == 0 && !adv) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flight_kernel<0, 0>, block, lds);
     else if (mode == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flight_kernel<0, 1>, block, lds);
     else if (!adv) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flight_kernel<1, 0>, block, lds);
@@ -734,6 +773,11 @@ extern "C" hipError_t dsmc_launch_phys(const DsmcEnv *E, const DsmcEnv *Ed, cons
         else
             hipLaunchKernelGGL((phys_kernel<true, 0>), dim3(grid), dim3(block), lds, st, none, (const DsmcEnv *)nullptr,
                                *P, rec, Q, p, svc_batch, step_budget);
+    } else if (E->mode == 0) {
+        if (!Ed || E->advanced) return hipErrorInvalidValue;
+        const size_t lds = sizeof(double) * (size_t)((E->ftd + 1) + 2 * (E->ffd + 1) + (E->hd + 2));
+        hipLaunchKernelGGL(compat_kernel, dim3(grid), dim3(block), lds, st, *E, Ed, rec, Q, p, svc_batch,
+                           step_budget);
     } else {
         if (E->mode != 1 || !Ed) return hipErrorInvalidValue;
         const size_t lds = sizeof(double) * (size_t)((E->ftd + 1) + 2 * (E->ffd + 1) + (E->hd + 2));

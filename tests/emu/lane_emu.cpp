@@ -70,31 +70,37 @@ static void run_one_split(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t 
     }
 }
 
-template <int ADV>
+template <int MODE, int ADV>
 static void run_one_inline(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t budget)
 {
     DsmcRec R;
     memset(&R, 0x5A, sizeof R);
     DsmcLane L;
     garbage(L);
-    lane_start<DsmcOpsHost, 1, ADV>(L, E, n, cnt);
+    lane_start<DsmcOpsHost, MODE, ADV>(L, E, n, cnt);
     if (L.phase != LANE_FLIGHT) return;
-    rec_store_all<1>(L, R, 0);
+    rec_store_all<MODE>(L, R, 0);
     DsmcCellTabs TB = {E.tq, E.hr, E.cs};
     for (;;) {
         DsmcLane F;
         garbage(F);
-        rec_load_phys(F, R);
+        if (MODE == 0) rec_load_all<0>(F, R);
+        else rec_load_phys(F, R);
         const int64_t end = F.steps + budget;
         for (;;) {
             bool paused = false;
-            lane_step<DsmcOpsHost, 1, ADV>(F, E, TB);
+            lane_step<DsmcOpsHost, MODE, ADV>(F, E, TB);
             if (F.phase == LANE_FLIGHT && F.steps >= end) paused = true;
             if (F.phase == LANE_LANDED || F.phase == LANE_ENDED || F.phase == LANE_COLLIDE) {
                 DsmcLane T = F;
-                lane_service<DsmcOpsHost, 1, ADV>(T, E, cnt);
+                lane_service<DsmcOpsHost, MODE, ADV>(T, E, cnt);
                 F = T;
                 if (F.phase == LANE_DONE) return;
+            }
+            if (paused && MODE == 0) {     /* compat: the visit is flushed */
+                lane_flush<DsmcOpsHost>(F, E);
+                rec_store_all<0>(F, R, 0);
+                break;
             }
             if (paused) {                  /* the open visit travels with the record */
                 rec_store_phys(F, R);
@@ -105,9 +111,10 @@ static void run_one_inline(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t
 }
 
 /* Advanced (collisional) compat run with fits supplied by the caller. */
-extern "C" int emu_advanced(const dsmc_params *p, const dsmc_surface *s, const double *copy_atm,
-                            int64_t atm_total, const int32_t *count_num, const double *coeffs,
-                            const double *pmin, const double *pmax, const int32_t *pvalid, dsmc_result *r)
+/* One particle at a time straight through the state machine. */
+static int advanced_run(const dsmc_params *p, const dsmc_surface *s, const double *copy_atm, int64_t atm_total,
+                        const int32_t *count_num, const double *coeffs, const double *pmin, const double *pmax,
+                        const int32_t *pvalid, dsmc_result *r)
 {
     DsmcEnv E;
     dsmc::HostTables T;
@@ -143,13 +150,20 @@ extern "C" int emu_advanced(const dsmc_params *p, const dsmc_surface *s, const d
     return 0;
 }
 
+extern "C" int emu_advanced(const dsmc_params *p, const dsmc_surface *s, const double *copy_atm,
+                            int64_t atm_total, const int32_t *count_num, const double *coeffs,
+                            const double *pmin, const double *pmax, const int32_t *pvalid, dsmc_result *r)
+{
+    return advanced_run(p, s, copy_atm, atm_total, count_num, coeffs, pmin, pmax, pvalid, r);
+}
+
 /* path 0: one particle at a time straight through the state machine; 1: the
  * split flight / service kernels' record traffic; 2: the physical flight kernel
  * with the service inline.  `budget`: steps per flight launch (paths 1, 2). */
 extern "C" int emu_ballistic_path(const dsmc_params *p, const dsmc_surface *s, dsmc_result *r, int path,
                                   int64_t budget)
 {
-    if (path < 0 || path > 2 || (path == 2 && p->mode != DSMC_MODE_PHYSICAL) || (path && budget < 1)) return -1;
+    if (path < 0 || path > 2 || (path && budget < 1)) return -1;
     DsmcEnv E;
     dsmc::HostTables T;
     int rc = dsmc::build_env(p, s, 0, &E, &T);
@@ -192,7 +206,8 @@ extern "C" int emu_ballistic_path(const dsmc_params *p, const dsmc_surface *s, d
         memset(&L, 0, sizeof L);
         if (path == 1 && p->mode == DSMC_MODE_COMPAT) run_one_split<0, 0>(E, n, cnt, budget);
         else if (path == 1) run_one_split<1, 0>(E, n, cnt, budget);
-        else if (path == 2) run_one_inline<0>(E, n, cnt, budget);
+        else if (path == 2 && p->mode == DSMC_MODE_COMPAT) run_one_inline<0, 0>(E, n, cnt, budget);
+        else if (path == 2) run_one_inline<1, 0>(E, n, cnt, budget);
         else if (p->mode == DSMC_MODE_COMPAT) run_one<0, 0>(L, E, n, cnt);
         else run_one<1, 0>(L, E, n, cnt);
     }
@@ -244,7 +259,7 @@ extern "C" int emu_physical_coll(const dsmc_params *p, const dsmc_surface *s, co
     for (int64_t n = 0; n < p->nparticles; n++) {
         DsmcLane L;
         memset(&L, 0, sizeof L);
-        if (path == 2) run_one_inline<1>(E, n, cnt, budget);
+        if (path == 2) run_one_inline<1, 1>(E, n, cnt, budget);
         else run_one<1, 1>(L, E, n, cnt);
     }
     if (r->density) for (size_t i = 0; i < cells; i++) r->density[i] = (double)vcount[i];

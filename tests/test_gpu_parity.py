@@ -102,6 +102,34 @@ def test_compat_ballistic_bitwise(lib, dev, tname, n):
     assert np.allclose(got.horizon, want.horizon, rtol=1e-12, atol=1e-15)
 
 
+@pytest.mark.parametrize("shape", [(8, 4096, 0), (8, 37, 0), (48, 37, 1), (16, 2147483647, 1)])
+def test_compat_ballistic_kernel_paths(lib, dev, shape):
+    """Compat ballistic on both kernel paths through staged runs with explicit
+    launch shapes: the split flight / service kernels and the flight loop with
+    the service inline (the default), each also with 37-step launches (every
+    particle paused and resumed many times): bit for bit as the oracle."""
+    T, ftd, ffd = read_T(GOLDEN / "T_thermal.dat")
+    p = compat_params(ftd, ffd, 300, 17)
+    surf = SurfaceArrays(T, ftd, ffd)
+    rb, budget, inline = shape
+    print(f"compat kernel path: refill/service batch {rb}, budget {budget}, inline {inline}", flush=True)
+    out = Buffers(p, 5)
+    st = C.c_int(0)
+    run = lib.dsmc_run_create(C.byref(p), C.byref(surf.surface), C.byref(out.result), C.byref(st))
+    assert run and st.value == 0
+    sh = capi.LaunchShape(256, 4, rb, budget, inline)
+    assert lib.dsmc_run_launch(run, C.byref(sh)) == 0 and lib.dsmc_run_sync(run) == 0
+    got = Buffers(p, 5)
+    assert lib.dsmc_run_collect(run, C.byref(got.result)) == 0
+    lib.dsmc_run_destroy(run)
+    want = run_oracle(dev, p, surf, nthreads=8)
+    for k in ("ttime", "steps", "end_state", "fate", "hops"):
+        assert np.array_equal(getattr(got, k), getattr(want, k)), k
+    assert got.counters() == want.counters()
+    assert np.allclose(got.density, want.density, rtol=1e-12, atol=1e-12)
+    assert np.allclose(got.horizon, want.horizon, rtol=1e-12, atol=1e-15)
+
+
 def test_physical_bitwise(lib, dev):
     p = capi.preset(lib, PRESET_CERES_PHYSICAL, 36, 72, n=3000, seed=5)
     surf = icosphere(lib, 3)

@@ -217,15 +217,17 @@ def test_record_paths_match_oracle(lib, emu, path, budget):
         assert np.array_equal(getattr(got, k), getattr(want, k)), k
     assert got.counters() == want.counters()
     assert want.hops.mean() > 20
-    if path == 1:
-        T, ftd, ffd = read_T(GOLDEN / "T_thermal.dat")
-        p = compat_params(ftd, ffd, 80, 5)
-        s = SurfaceArrays(T, ftd, ffd)
-        want = run_oracle(dev, p, s, nthreads=4)
-        got = Buffers(p, 5)
-        assert emu.emu_ballistic_path(C.byref(p), C.byref(s.surface), C.byref(got.result), 1, budget) == 0
-        for k in ("ttime", "steps", "end_state", "fate", "hops"):
-            assert np.array_equal(getattr(got, k), getattr(want, k)), k
+    # compat runs on both kernel paths: split, and the flight loop with the service inline
+    T, ftd, ffd = read_T(GOLDEN / "T_thermal.dat")
+    p = compat_params(ftd, ffd, 80, 5)
+    s = SurfaceArrays(T, ftd, ffd)
+    want = run_oracle(dev, p, s, nthreads=4)
+    got = Buffers(p, 5)
+    assert emu.emu_ballistic_path(C.byref(p), C.byref(s.surface), C.byref(got.result), path, budget) == 0
+    for k in ("ttime", "steps", "end_state", "fate", "hops"):
+        assert np.array_equal(getattr(got, k), getattr(want, k)), k
+    assert got.counters() == want.counters()
+    assert np.allclose(got.density, want.density, rtol=1e-12, atol=1e-12)
 
 
 @pytest.mark.parametrize("body", range(4))
@@ -351,3 +353,4 @@ def test_lane_state_machine_matches_oracle_advanced(emu):
     for k in ("ttime", "steps", "end_state", "fate", "hops"):
         assert np.array_equal(getattr(got, k), getattr(want, k)), k
     assert got.counters() == want.counters() and want.result.n_collisions > 100
+
