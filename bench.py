@@ -77,6 +77,44 @@ def cpu_baseline(p, surf, sample: int, threads: int) -> dict:
                       f"{dt:.2f} s, oracle/libcpu_ref.so, OpenMP schedule(dynamic,100))"}
 
 
+THERMAL_GRIDS = ((36, 72), (360, 720))
+
+
+def thermal_run(lib) -> dict:
+    """The thermal model (thermal_model.c thermal_solve, default parameters: 467
+    depth points, 5 substeps) on the device at the reference's 36x72 grid and at
+    360x720; wall time of dsmc_thermal_solve (upload, kernel, result back)."""
+    tp = capi.ThermalParams()
+    lib.thermal_params_default(C.byref(tp))
+    out = {"depth_points": 467, "substeps": int(tp.n_substep)}
+    for ftd, ffd in THERMAL_GRIDS:
+        T = np.zeros(ftd * ffd)
+        ms = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            if lib.dsmc_thermal_solve(ftd, ffd, C.byref(tp), None, T.ctypes.data_as(C.POINTER(C.c_double))):
+                raise RuntimeError("dsmc_thermal_solve failed")
+            ms.append((time.perf_counter() - t0) * 1e3)
+        out[f"{ftd}x{ffd}_ms"] = round(min(ms), 3)
+        out[f"{ftd}x{ffd}_cells_per_s"] = ftd * ffd / (min(ms) / 1e3)
+    return out
+
+
+def thermal_cpu(threads: int) -> dict:
+    """The oracle's thermal model on the host cores, same grids."""
+    from tests.helpers import oracle
+    lib = oracle(devmath=False)
+    tp = capi.ThermalParams()
+    lib.cpu_ref_thermal_params_default(C.byref(tp))
+    out = {"cores": threads}
+    for ftd, ffd in THERMAL_GRIDS:
+        T = np.zeros(ftd * ffd)
+        t0 = time.perf_counter()
+        lib.cpu_ref_thermal(ftd, ffd, C.byref(tp), None, T.ctypes.data_as(C.POINTER(C.c_double)), threads)
+        out[f"{ftd}x{ffd}_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+    return out
+
+
 def plain_run(lib, surf, n, args, shape) -> dict:
     """Stages one plain run of n particles and times args.steps complete executions."""
     p = capi.preset(lib, PRESET_CERES_PHYSICAL, 36, 72, n=n, seed=args.seed)
@@ -417,6 +455,7 @@ def main():
     ap.add_argument("--serve-collisional", type=float, default=0.3, help="fraction of collisional requests")
     ap.add_argument("--no-coll", action="store_true")
     ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in run_ballistic files leg")
+    ap.add_argument("--no-thermal", action="store_true", help="skip the thermal model leg")
     ap.add_argument("--coll-particles", type=int, default=2_000_000)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only copy a buffer of this many MiB device to device 5 times (HBM counter calibration)")
@@ -460,6 +499,9 @@ def main():
     if not args.no_dropin:
         progress("drop-in run_ballistic work at 1e4 particles (device run + .dat files)")
         line["dropin_files"] = dropin_run(lib, args)
+    if not args.no_thermal:
+        progress("thermal model (36x72, 360x720)")
+        line["thermal"] = thermal_run(lib)
     tuned = None
     if not args.no_batch:
         progress("batched: 64 runs (autotune, launches, plain sequential)")
@@ -491,6 +533,8 @@ def main():
         progress("CPU baseline")
         threads = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(p, surf, args.cpu_sample, threads)
+        if not args.no_thermal:
+            line["cpu_baseline"]["thermal"] = thermal_cpu(threads)
     print(json.dumps(line))
 
 

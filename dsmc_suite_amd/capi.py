@@ -99,6 +99,21 @@ def ptr(a: np.ndarray | None, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
 
 
+class ThermalParams(C.Structure):
+    """dsmc_thermal_params (the reference's ThermalParams, include/thermal.h)."""
+    _fields_ = [("density", C.c_double), ("conductivity", C.c_double), ("Cp", C.c_double),
+                ("albedo", C.c_double), ("distance_AU", C.c_double), ("depth", C.c_double),
+                ("n_substep", C.c_int), ("z_angle", C.c_double), ("p_mat", C.c_double * 5),
+                ("LH_mat", C.c_double * 5), ("tran_unit_p", C.c_double), ("tran_unit_LH", C.c_double)]
+
+
+class SphGrid(C.Structure):
+    """dsmc_sph_grid (the reference's SphericalGrid, include/types.h:56-64)."""
+    _fields_ = [("ftd", C.c_int), ("ffd", C.c_int), ("hd", C.c_int), ("density", C.POINTER(C.c_double)),
+                ("temperature", C.POINTER(C.c_double)), ("volume", C.POINTER(C.c_double)),
+                ("horizon", C.POINTER(C.c_double))]
+
+
 class Buffers:
     """Host arrays backing a Result; keeps them alive and exposes numpy views."""
 

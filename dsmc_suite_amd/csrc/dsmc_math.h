@@ -230,4 +230,13 @@ DM_FN double dm_log(double x)
     return fma(de, DM_LN2_HI32, fma(de, DM_LN2_LO32, logm));
 }
 
+/* ---------------------------------------------------------------- log10 / 10^y / x^(1/4) */
+/* log10 x = log x / ln 10; 10^y = exp(y ln 10) (y ln 10 rounded once: a relative
+ * error of about |y| ln 10 ulp, below 1e-14 for the vapour pressures of the
+ * thermal model); x^(1/4) = sqrt(sqrt(x)). */
+#define DM_LN10 0x1.26bb1bbb55516p+1
+DM_FN double dm_log10(double x) { return dm_log(x) / DM_LN10; }
+DM_FN double dm_pow10(double y) { return dm_exp(y * DM_LN10); }
+DM_FN double dm_root4(double x) { return sqrt(sqrt(x)); }
+
 #endif /* DSMC_MATH_H */

@@ -369,6 +369,44 @@ int write_horizon_dat(const char *filename, const double *horizon, int ftd, int 
  * counterpart in the reference, whose file functions are serial. */
 int dsmc_io_threads(int n);
 
+/* ------------------------------------------------------------------ thermal model */
+/* src/thermal/thermal_model.c: per surface cell (td, fd) a 1-D heat equation in
+ * depth (Crank-Nicolson, insulating bottom), the surface temperature found by
+ * bisection on the energy balance (absorbed flux + conduction - emission -
+ * sublimation) before each substep; on the device, one lane per cell.
+ *
+ * dsmc_thermal_params: field for field the reference's ThermalParams
+ * (include/thermal.h), so its callers link against this library unchanged. */
+typedef struct {
+    double density;       /* regolith density [kg/m^3] */
+    double conductivity;  /* [W/m/K] */
+    double Cp;            /* specific heat [J/kg/K] */
+    double albedo;
+    double distance_AU;
+    double depth;         /* [m] */
+    int n_substep;        /* substeps per rotation step */
+    double z_angle;       /* obliquity [rad] (not read by the solve) */
+    double p_mat[5];      /* Antoine vapour pressure coefficients */
+    double LH_mat[5];     /* latent heat coefficients */
+    double tran_unit_p;   /* Torr to Pa */
+    double tran_unit_LH;  /* cal to J */
+} dsmc_thermal_params;
+
+/* The reference's SphericalGrid (include/types.h:56-64), same layout. */
+typedef struct {
+    int ftd, ffd, hd;
+    double *density, *temperature, *volume, *horizon;
+} dsmc_sph_grid;
+
+/* thermal_model.c:23-49 */
+void thermal_params_default(dsmc_thermal_params *tp);
+/* thermal_model.c:105-283: fills grid->temperature [ftd*ffd]; flux [ftd*ffd]
+ * absorbed solar flux or NULL for the built-in cos(lat) sin(lon) day side.
+ * Prints the reference's three progress lines; nthreads is unused (device). */
+void thermal_solve(dsmc_sph_grid *grid, const dsmc_thermal_params *tp, const double *flux, int nthreads);
+/* The same with a status: T_surface [ftd*ffd]; 0 or a dsmc_status. */
+int dsmc_thermal_solve(int ftd, int ffd, const dsmc_thermal_params *tp, const double *flux, double *T_surface);
+
 /* ------------------------------------------------------------------ misc */
 const char *dsmc_version(void);
 int dsmc_device_info(char *buf, size_t len);   /* "gfx950, 256 CUs, ..." */

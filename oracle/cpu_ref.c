@@ -37,6 +37,8 @@
  * wrapping (tests/golden/make_golden.sh).  Physical mode has no reference
  * counterpart: it is pinned by invariants (energy drift, launch histogram
  * against the flux weights, emission moments) - see tests/test_oracle_physical.py.
+ * The thermal model (src/thermal/thermal_model.c) is pinned bit for bit against
+ * the compiled reference (tests/test_thermal.py, oracle/ref_thermal.c).
  */
 #include <math.h>
 #include <stdio.h>
@@ -59,6 +61,9 @@
 #define OM_COS dm_cos
 #define OM_EXP dm_exp
 #define OM_LOG dm_log
+#define OM_LOG10 dm_log10
+#define OM_POW10 dm_pow10
+#define OM_ROOT4 dm_root4
 #else
 #define OM_ASIN asin
 #define OM_ATAN2 atan2
@@ -66,6 +71,9 @@
 #define OM_COS cos
 #define OM_EXP exp
 #define OM_LOG log
+#define OM_LOG10 log10
+#define OM_POW10(y) pow(10.0, (y))
+#define OM_ROOT4(x) pow((x), 0.25)
 #endif
 
 enum { ORNG_KEYED = 0, ORNG_SEQUENTIAL = 1, ORNG_RAND_R = 2 };
@@ -1313,6 +1321,122 @@ int cpu_ref_collisional(const dsmc_params *p1, const dsmc_params *p2, const dsmc
 }
 
 /* Number of OpenMP threads this build would use by default. */
+/* ------------------------------------------------------------------ thermal model */
+/* src/thermal/thermal_model.c restated.  Constants of include/constants.h. */
+#define TH_K_B 1.38e-23
+#define TH_MMOL (18e-3 / 6.02e23)
+#define TH_SIGMA 5.6704e-8
+#define TH_SOLAR 1368.0
+#define TH_PERIOD (9.07417 * 3600.0)
+
+/* thermal_model.c:23-49 */
+void cpu_ref_thermal_params_default(dsmc_thermal_params *tp)
+{
+    memset(tp, 0, sizeof(*tp));
+    tp->density = 730.0;
+    tp->conductivity = 6e-4;
+    tp->Cp = 1300.0;
+    tp->albedo = 0.02;
+    tp->distance_AU = 2.6;
+    tp->depth = 0.05;
+    tp->n_substep = 5;
+    tp->z_angle = 12.3 / 180.0 * M_PI;
+    const double p[5] = {-2445.5646, -6.757169, -0.01677006, 1.20514e-5, 8.2312};
+    const double lh[5] = {0.0, 12420.0, -4.8, 0.0, 0.0};
+    for (int i = 0; i < 5; i++) { tp->p_mat[i] = p[i]; tp->LH_mat[i] = lh[i]; }
+    tp->tran_unit_p = 133.32237;
+    tp->tran_unit_LH = 4.2;
+}
+
+/* thermal_model.c:52-75: vapour pressure, latent heat, surface balance */
+static double th_balance(double T, double F, double k_over_dr, double oT, const dsmc_thermal_params *tp)
+{
+    const double *a = tp->p_mat, *b = tp->LH_mat;
+    double lg = a[0] / T + a[1] + a[2] * T + a[3] * T * T + a[4] * OM_LOG10(T);
+    double p = tp->tran_unit_p * OM_POW10(lg);
+    double LH = tp->tran_unit_LH * (b[0] / T + b[1] + b[2] * T + b[3] * T * T + b[4] * OM_LOG10(T));
+    double sublim = LH * sqrt(TH_MMOL / (2.0 * M_PI * TH_K_B * T)) * p;
+    return F + k_over_dr * oT - k_over_dr * T - TH_SIGMA * T * T * T * T - sublim;
+}
+
+/* thermal_model.c:105-283 for every cell: T_out[td*ffd+fd] = surface temperature.
+ * Returns the number of depth points, or -1 on bad arguments. */
+int cpu_ref_thermal(int ftd, int ffd, const dsmc_thermal_params *tp, const double *flux, double *T_out,
+                    int nthreads)
+{
+    if (ftd <= 0 || ffd <= 0 || !tp || tp->n_substep <= 0 || !T_out) return -1;
+    const int n_sub = tp->n_substep;
+    const double dt = TH_PERIOD / ffd / n_sub;
+    const double kk = tp->conductivity, lo = tp->density, Cp = tp->Cp;
+    const double dr = sqrt(0.1 * 2.0 * dt * kk / lo / Cp);
+    int h = (int)ceil(tp->depth / dr);
+    if (h < 3) h = 3;
+    const double R = kk * dt / lo / 2.0 / Cp / (dr * dr);
+    const double k_over_dr = kk / dr;
+    const size_t nc = (size_t)ftd * ffd;
+    double *F = (double *)malloc(nc * sizeof(double));
+    if (flux) memcpy(F, flux, nc * sizeof(double));
+    else {
+        const double Si = TH_SOLAR * (1.0 / tp->distance_AU) * (1.0 / tp->distance_AU);
+        const double dF = Si * (1.0 - tp->albedo);
+        for (int td = 0; td < ftd; td++)
+            for (int fd = 0; fd < ffd; fd++) {
+                double thi = (td + 0.5) / ftd * M_PI - M_PI / 2.0;
+                double fi = (fd + 0.5) / ffd * 2.0 * M_PI;
+                F[(size_t)td * ffd + fd] = fi >= M_PI ? 0.0 : dF * OM_COS(thi) * OM_SIN(fi);
+            }
+    }
+    /* the tridiagonal rows: 0 fixed (surface), 1..h-2 implicit CN, h-1 zero flux */
+    double *a = (double *)malloc(h * sizeof(double)), *b = (double *)malloc(h * sizeof(double));
+    double *c = (double *)malloc(h * sizeof(double));
+    a[0] = 0.0; b[0] = 1.0; c[0] = 0.0;
+    for (int i = 1; i < h - 1; i++) { a[i] = -R; b[i] = 1.0 + 2.0 * R; c[i] = -R; }
+    a[h - 1] = -1.0 / dr; b[h - 1] = 1.0 / dr; c[h - 1] = 0.0;
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+    {
+        double *q = (double *)malloc(h * sizeof(double)), *d = (double *)malloc(h * sizeof(double));
+        double *cp = (double *)malloc(h * sizeof(double)), *dp = (double *)malloc(h * sizeof(double));
+#pragma omp for schedule(dynamic)
+        for (long cell = 0; cell < (long)nc; cell++) {
+            const double Fc = fabs(F[cell]);
+            const double T_init = Fc > 0 ? OM_ROOT4(Fc / TH_SIGMA) : 100.0;
+            for (int i = 0; i < h; i++) q[i] = T_init;
+            for (int hh = 0; hh < n_sub; hh++) {
+                const double oT = q[1];
+                double T0 = 50.0, T1 = 300.0;
+                double f0 = th_balance(T0, Fc, k_over_dr, oT, tp), f1 = th_balance(T1, Fc, k_over_dr, oT, tp);
+                for (int m = 0; m < 30; m++) {
+                    if (T1 >= 10000.0) break;
+                    if (f0 * f1 > 0) { T1 *= 5.0; f1 = th_balance(T1, Fc, k_over_dr, oT, tp); continue; }
+                    double T2 = (T0 + T1) / 2.0;
+                    double f2 = th_balance(T2, Fc, k_over_dr, oT, tp);
+                    if (f0 * f2 <= 0) { T1 = T2; f1 = f2; }
+                    else { T0 = T2; f0 = f2; }
+                    if (fabs(T0 - T1) <= 0.01) break;
+                }
+                q[0] = (T0 + T1) / 2.0;
+                d[0] = q[0];
+                for (int i = 1; i < h - 1; i++) d[i] = R * q[i - 1] + (1.0 - 2.0 * R) * q[i] + R * q[i + 1];
+                d[h - 1] = 0.0;
+                /* Thomas algorithm (thermal_model.c:82-103) */
+                cp[0] = c[0] / b[0];
+                dp[0] = d[0] / b[0];
+                for (int i = 1; i < h; i++) {
+                    double den = b[i] - a[i] * cp[i - 1];
+                    cp[i] = c[i] / den;
+                    dp[i] = (d[i] - a[i] * dp[i - 1]) / den;
+                }
+                q[h - 1] = dp[h - 1];
+                for (int i = h - 2; i >= 0; i--) q[i] = dp[i] - cp[i] * q[i + 1];
+            }
+            T_out[cell] = q[0];
+        }
+        free(q); free(d); free(cp); free(dp);
+    }
+    free(a); free(b); free(c); free(F);
+    return h;
+}
+
 int cpu_ref_max_threads(void) { return omp_get_max_threads(); }
 
 /* ------------------------------------------------------------------ unit-level exports */

@@ -1,0 +1,143 @@
+"""Thermal model (src/thermal/thermal_model.c): oracle, reference and device.
+
+* the oracle's libm build equals the compiled reference (thermal_model.c and
+  grid.c unmodified, oracle/ref_thermal.c) bit for bit, for the default
+  parameters and for others (depth grid, substeps, distance, external flux);
+* its device-math build (the transcendentals the kernel evaluates) stays within
+  the bisection's 0.01 K bracket of it;
+* the HIP kernel (GPU) equals the device-math build bit for bit, and the
+  drop-in thermal_solve fills a SphericalGrid the same way.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from dsmc_suite_amd import capi
+from dsmc_suite_amd.capi import SphGrid, ThermalParams
+from tests.helpers import ROOT, oracle
+
+REF_SO = ROOT / "oracle" / "_ref" / "libref_thermal.so"
+DP = C.POINTER(C.c_double)
+
+
+def defaults(lib_oracle) -> ThermalParams:
+    tp = ThermalParams()
+    lib_oracle.cpu_ref_thermal_params_default(C.byref(tp))
+    return tp
+
+
+def variants(lib_oracle):
+    """(name, params, flux or None, ftd, ffd)"""
+    out = [("default", defaults(lib_oracle), None, 36, 72)]
+    tp = defaults(lib_oracle)
+    tp.depth, tp.n_substep = 0.01, 3
+    out.append(("shallow, 3 substeps", tp, None, 18, 36))
+    tp = defaults(lib_oracle)
+    tp.distance_AU, tp.albedo, tp.n_substep = 1.0, 0.1, 8
+    out.append(("1 AU, 8 substeps", tp, None, 12, 24))
+    rng = np.random.default_rng(3)
+    flux = rng.uniform(-400, 900, 20 * 30)
+    flux[::9] = 0.0
+    out.append(("external flux", defaults(lib_oracle), flux, 20, 30))
+    return out
+
+
+def run_oracle_thermal(lib, tp, flux, ftd, ffd):
+    out = np.zeros(ftd * ffd)
+    f = None if flux is None else np.ascontiguousarray(flux).ctypes.data_as(DP)
+    h = lib.cpu_ref_thermal(ftd, ffd, C.byref(tp), f, out.ctypes.data_as(DP), 8)
+    assert h >= 3
+    return out
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return oracle(devmath=False)
+
+
+@pytest.fixture(scope="module")
+def dev():
+    return oracle(devmath=True)
+
+
+def test_oracle_thermal_equals_the_reference(orc):
+    if not REF_SO.exists():
+        pytest.skip("the compiled reference (make -C oracle ref) is not built here")
+    ref = C.CDLL(str(REF_SO))
+    tp0 = ThermalParams()
+    ref.thermal_params_default(C.byref(tp0))
+    mine = defaults(orc)
+    assert bytes(tp0) == bytes(mine)                                   # thermal_model.c:23-49
+    for name, tp, flux, ftd, ffd in variants(orc):
+        got = run_oracle_thermal(orc, tp, flux, ftd, ffd)
+        want = np.zeros(ftd * ffd)
+        f = None if flux is None else np.ascontiguousarray(flux).ctypes.data_as(DP)
+        assert ref.ref_thermal(ftd, ffd, C.byref(tp), f, want.ctypes.data_as(DP), 4) == 0
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), name
+        assert np.all((got > 40) & (got < 400)), name
+
+
+def test_thermal_physics(orc):
+    """What the mathematics fixes: night cells (no flux) cool from their 100 K
+    start only by conduction and sublimation; day cells approach the radiative
+    equilibrium (F / sigma)^(1/4) from below (sublimation takes heat); hotter
+    for more flux; no latitude asymmetry for a symmetric flux field."""
+    tp = defaults(orc)
+    T = run_oracle_thermal(orc, tp, None, 36, 72).reshape(36, 72)
+    assert np.all(T[:, 36:] <= 100.0 + 0.01) and np.all(T[:, 36:] > 50.0)
+    dF = 1368.0 / 2.6 ** 2 * (1 - tp.albedo)
+    lat = (np.arange(36) + 0.5) / 36 * np.pi - np.pi / 2
+    lon = (np.arange(72) + 0.5) / 72 * 2 * np.pi
+    F = dF * np.outer(np.cos(lat), np.sin(lon))[:, :36]
+    Teq = (F / 5.6704e-8) ** 0.25
+    assert np.all(T[:, :36] <= Teq + 0.01)
+    assert np.allclose(T[:, :36], T[::-1, :36], atol=0.02)               # north-south symmetric
+    assert T[18, 18] > T[18, 2] and T[18, 18] > T[2, 18]
+
+
+def test_devmath_build_within_the_bracket(orc, dev):
+    for name, tp, flux, ftd, ffd in variants(orc):
+        a = run_oracle_thermal(orc, tp, flux, ftd, ffd)
+        b = run_oracle_thermal(dev, tp, flux, ftd, ffd)
+        assert np.max(np.abs(a - b)) <= 0.01, name
+
+
+def thermal_lib():
+    lib = capi.load()
+    lib.dsmc_thermal_solve.argtypes = [C.c_int, C.c_int, C.c_void_p, DP, DP]
+    lib.thermal_solve.argtypes = [C.POINTER(SphGrid), C.c_void_p, DP, C.c_int]
+    lib.thermal_params_default.argtypes = [C.c_void_p]
+    return lib
+
+
+def test_library_defaults_and_arguments(orc):
+    lib = thermal_lib()
+    tp = ThermalParams()
+    lib.thermal_params_default(C.byref(tp))
+    assert bytes(tp) == bytes(defaults(orc))
+    out = np.zeros(4)
+    assert lib.dsmc_thermal_solve(0, 72, C.byref(tp), None, out.ctypes.data_as(DP)) == -1
+    bad = ThermalParams.from_buffer_copy(bytes(tp))
+    bad.n_substep = 0
+    assert lib.dsmc_thermal_solve(2, 2, C.byref(bad), None, out.ctypes.data_as(DP)) == -1
+
+
+@pytest.mark.gpu
+def test_gpu_thermal_bitwise(orc, dev):
+    lib = thermal_lib()
+    for name, tp, flux, ftd, ffd in variants(orc):
+        print(f"thermal kernel: {name} {ftd}x{ffd}", flush=True)
+        got = np.zeros(ftd * ffd)
+        f = None if flux is None else np.ascontiguousarray(flux).ctypes.data_as(DP)
+        assert lib.dsmc_thermal_solve(ftd, ffd, C.byref(tp), f, got.ctypes.data_as(DP)) == 0, name
+        want = run_oracle_thermal(dev, tp, flux, ftd, ffd)
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), name
+    # the drop-in entry point fills the grid's temperature field
+    tp = defaults(orc)
+    temp = np.zeros(36 * 72)
+    g = SphGrid(36, 72, 1, None, temp.ctypes.data_as(DP), None, None)
+    lib.thermal_solve(C.byref(g), C.byref(tp), None, 4)
+    assert np.array_equal(temp.view(np.uint64), run_oracle_thermal(dev, tp, None, 36, 72).view(np.uint64))
