@@ -527,15 +527,20 @@ static uint64_t env_seed()
  * the threads of their file work; the device runs the particles. */
 static int driver_threads(int nthreads) { return nthreads < 1 ? 1 : (nthreads > 64 ? 64 : nthreads); }
 
-extern "C" void run_ballistic(int totaln, const char *temp_file, const char *output_prefix, int nthreads)
+/* What a ballistic run hands to the file writers (and, in the pipeline, to
+ * phase 2). */
+struct BallisticOut {
+    int ftd = 0, ffd = 0, hd = 0;
+    int64_t nrec = 0;
+    std::vector<double> den, vrec;
+    std::vector<int32_t> vtd, vfd, vht;
+};
+
+/* run_ballistic from a temperature field in memory: the run, its messages and
+ * its four files (ballistic.c:35-309).  Returns 0 or a dsmc_status. */
+static int ballistic_core(int totaln, const double *T, int ftd, int ffd, const char *output_prefix, int io,
+                          BallisticOut *keep)
 {
-    const int io = driver_threads(nthreads);
-    int ftd, ffd;
-    double *T = read_temperature_dat(temp_file, &ftd, &ffd);
-    if (!T) {
-        fprintf(stderr, "Failed to read temperature file\n");
-        return;
-    }
     dsmc_params p;
     dsmc_params_preset(&p, DSMC_PRESET_COMPAT_BALLISTIC, ftd, ffd);
     p.nparticles = totaln < 0 ? 0 : totaln;
@@ -557,10 +562,9 @@ extern "C" void run_ballistic(int totaln, const char *temp_file, const char *out
     printf("  Escape velocity = %.2f m/s\n", p.v_esc);
     printf("  Lifetime = %.1f s\n", p.lifetime);
     int rc = dsmc_run_ballistic(&p, &s, &r);
-    free(T);
     if (rc) {
         fprintf(stderr, "dsmc_run_ballistic failed (%d)\n", rc);
-        return;
+        return rc;
     }
     printf("Simulation complete. Escaped: %d/%d\n", (int)r.n_escaped, totaln);
     std::string pre(output_prefix);
@@ -569,6 +573,57 @@ extern "C" void run_ballistic(int totaln, const char *temp_file, const char *out
     write_v_record_n((pre + "_v_record.dat").c_str(), vrec.data(), vtd.data(), vfd.data(), vht.data(),
                      (int)r.vrec_n, io);
     write_ttime_n((pre + "_ttime.dat").c_str(), ttime.data(), 5, (int)p.nparticles, io);
+    if (keep) {
+        keep->ftd = ftd; keep->ffd = ffd; keep->hd = hd; keep->nrec = r.vrec_n;
+        keep->den.swap(den);
+        vrec.resize((size_t)r.vrec_n * 3); vtd.resize(r.vrec_n); vfd.resize(r.vrec_n); vht.resize(r.vrec_n);
+        keep->vrec.swap(vrec); keep->vtd.swap(vtd); keep->vfd.swap(vfd); keep->vht.swap(vht);
+    }
+    return DSMC_OK;
+}
+
+extern "C" void run_ballistic(int totaln, const char *temp_file, const char *output_prefix, int nthreads)
+{
+    int ftd, ffd;
+    double *T = read_temperature_dat(temp_file, &ftd, &ffd);
+    if (!T) {
+        fprintf(stderr, "Failed to read temperature file\n");
+        return;
+    }
+    (void)ballistic_core(totaln, T, ftd, ffd, output_prefix, driver_threads(nthreads), nullptr);
+    free(T);
+}
+
+/* run_advanced with its inputs in memory (advanced.c:119-476): the run, its
+ * messages and its three files. */
+static int advanced_core(int totaln, const double *T, int ftd, int ffd, const dsmc_phase2_input *in,
+                         const char *output_prefix, int io)
+{
+    dsmc_params p;
+    dsmc_params_preset(&p, DSMC_PRESET_COMPAT_ADVANCED, ftd, ffd);
+    p.nparticles = totaln < 0 ? 0 : totaln;
+    p.seed = env_seed();
+    dsmc_surface s;
+    memset(&s, 0, sizeof s);
+    s.ftd = ftd; s.ffd = ffd; s.T_latlon = T;
+    const int hd = p.hd;
+    std::vector<double> den((size_t)hd * ftd * ffd), hor((size_t)ftd * ffd), ttime((size_t)9 * p.nparticles);
+    dsmc_result r;
+    memset(&r, 0, sizeof r);
+    r.density = den.data(); r.horizon = hor.data(); r.ttime = ttime.data();
+    printf("Building polynomial velocity fits...\n");
+    int rc = dsmc_run_advanced(&p, &s, in, &r);
+    if (rc) {
+        fprintf(stderr, "dsmc_run_advanced failed (%d)\n", rc);
+        return rc;
+    }
+    printf("Advanced DSMC complete. Escaped: %d/%d, Total collisions: %d\n", (int)r.n_escaped, totaln,
+           (int)r.n_collisions);
+    std::string pre(output_prefix);
+    write_density_n((pre + "_den.dat").c_str(), den.data(), ftd, ffd, hd, io);
+    write_horizon_n((pre + "_horizon.dat").c_str(), hor.data(), ftd, ffd, io);
+    write_ttime_n((pre + "_ttime.dat").c_str(), ttime.data(), 9, (int)p.nparticles, io);
+    return DSMC_OK;
 }
 
 extern "C" void run_advanced(int totaln, const char *temp_file, const char *prev_den_file,
@@ -581,13 +636,6 @@ extern "C" void run_advanced(int totaln, const char *temp_file, const char *prev
         fprintf(stderr, "Failed to read temperature file\n");
         return;
     }
-    dsmc_params p;
-    dsmc_params_preset(&p, DSMC_PRESET_COMPAT_ADVANCED, ftd, ffd);
-    p.nparticles = totaln < 0 ? 0 : totaln;
-    p.seed = env_seed();
-    dsmc_surface s;
-    memset(&s, 0, sizeof s);
-    s.ftd = ftd; s.ffd = ffd; s.T_latlon = T;
     dsmc_phase2_input in;
     memset(&in, 0, sizeof in);
     int pf = 0, pd = 0, ph = 0;
@@ -599,22 +647,58 @@ extern "C" void run_advanced(int totaln, const char *temp_file, const char *prev
         printf("Read %d velocity records\n", nrec);
         in.vrec = vr; in.vrec_td = vtd; in.vrec_fd = vfd; in.vrec_ht = vht; in.nrec = nrec;
     }
-    const int hd = p.hd;
-    std::vector<double> den((size_t)hd * ftd * ffd), hor((size_t)ftd * ffd), ttime((size_t)9 * p.nparticles);
-    dsmc_result r;
-    memset(&r, 0, sizeof r);
-    r.density = den.data(); r.horizon = hor.data(); r.ttime = ttime.data();
-    printf("Building polynomial velocity fits...\n");
-    int rc = dsmc_run_advanced(&p, &s, &in, &r);
+    (void)advanced_core(totaln, T, ftd, ffd, &in, output_prefix, io);
     free(T); free(atm); free(vr); free(vtd); free(vfd); free(vht);
-    if (rc) {
-        fprintf(stderr, "dsmc_run_advanced failed (%d)\n", rc);
-        return;
-    }
-    printf("Advanced DSMC complete. Escaped: %d/%d, Total collisions: %d\n", (int)r.n_escaped, totaln,
-           (int)r.n_collisions);
-    std::string pre(output_prefix);
-    write_density_n((pre + "_den.dat").c_str(), den.data(), ftd, ffd, hd, io);
-    write_horizon_n((pre + "_horizon.dat").c_str(), hor.data(), ftd, ffd, io);
-    write_ttime_n((pre + "_ttime.dat").c_str(), ttime.data(), 9, (int)p.nparticles, io);
+}
+
+/* ------------------------------------------------------------------ pipeline */
+/* v as the file holds it and a reader gets it back: the "%<prec>f" text
+ * (fixed_fmt / snprintf, as the writers) parsed as the readers parse it. */
+static double through_text(double v, int prec)
+{
+    std::string t;
+    append_fixed(t, v, prec, 0);
+    char *end;
+    return fast_strtod(t.c_str(), &end);
+}
+
+extern "C" int dsmc_pipeline(int totaln, const char *output_prefix, int nthreads)
+{
+    if (!output_prefix) return DSMC_ERR_ARG;
+    const int io = driver_threads(nthreads);
+    const std::string pre(output_prefix);
+    /* the thermal model on the reference CLI's 36 x 72 grid (main.c:166-187) */
+    const int ftd = 36, ffd = 72;
+    dsmc_thermal_params tp;
+    thermal_params_default(&tp);
+    std::vector<double> T((size_t)ftd * ffd);
+    dsmc_sph_grid g;
+    memset(&g, 0, sizeof g);
+    g.ftd = ftd; g.ffd = ffd; g.hd = 1; g.temperature = T.data();
+    thermal_solve(&g, &tp, nullptr, nthreads);
+    const std::string tfile = pre + "_temperature.dat";
+    FILE *fp = fopen(tfile.c_str(), "w");
+    if (!fp) return DSMC_ERR_IO;
+    fprintf(fp, "%d\n%d\n", ftd, ffd);
+    int rc = write_values(fp, 8, 6, T.data(), T.size(), io);
+    if (fclose(fp) || rc) return DSMC_ERR_IO;
+    for (double &v : T) v = through_text(v, 6);
+    /* phase 1 (main.c:189-193) */
+    BallisticOut b;
+    if ((rc = ballistic_core(totaln, T.data(), ftd, ffd, (pre + "_phase1").c_str(), io, &b))) return rc;
+    /* phase 2 on phase 1's outputs as its files hold them (main.c:195-202) */
+    std::vector<double> atm(b.den.size()), vr(b.vrec.size());
+    for_chunks(atm.size(), io, 16384, [&](size_t, size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; i++) atm[i] = through_text(b.den[i], 6);
+    });
+    for_chunks(vr.size(), io, 16384, [&](size_t, size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; i++) vr[i] = through_text(b.vrec[i], 6);
+    });
+    dsmc_phase2_input in;
+    memset(&in, 0, sizeof in);
+    in.copy_atm = atm.data(); in.atm_ftd = b.ftd; in.atm_ffd = b.ffd; in.atm_hd = b.hd;
+    printf("Read %d velocity records\n", (int)b.nrec);
+    in.vrec = vr.data(); in.vrec_td = b.vtd.data(); in.vrec_fd = b.vfd.data(); in.vrec_ht = b.vht.data();
+    in.nrec = b.nrec;
+    return advanced_core(totaln, T.data(), ftd, ffd, &in, (pre + "_phase2").c_str(), io);
 }

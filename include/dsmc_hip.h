@@ -349,6 +349,15 @@ void run_advanced(int totaln, const char *temp_file, const char *prev_den_file,
                   const char *prev_v_record_file, const char *output_prefix,
                   int nthreads);
 
+/* The CLI's `pipeline` command (src/main.c:153-204) without its ray-tracing
+ * step: the thermal model on 36 x 72 cells written to <prefix>_temperature.dat,
+ * then run_ballistic into <prefix>_phase1_*.dat and run_advanced on those into
+ * <prefix>_phase2_*.dat.  The same files and messages as those three steps;
+ * each phase takes the previous one's outputs from memory, passed through the
+ * same text conversion as the files (so it sees the values the files hold)
+ * instead of reading the files back.  Returns 0 or a dsmc_status. */
+int dsmc_pipeline(int totaln, const char *output_prefix, int nthreads);
+
 /* include/fileio.h:8-44, same semantics and printf formats */
 double *read_density_dat(const char *filename, int *ftd, int *ffd, int *hd, int has_hd);
 int write_density_dat(const char *filename, const double *den, int ftd, int ffd, int hd);

@@ -215,6 +215,54 @@ def test_dropin_run_ballistic_files(lib, tmp_path):
     assert int(vr[0]) > 0
 
 
+def run_python(code: str, seed: int):
+    r = subprocess.run([sys.executable, "-c", f"import ctypes as C, sys; sys.path.insert(0, "
+                        f"{str(capi.PKG_DIR.parent)!r}); from dsmc_suite_amd import capi; lib = capi.load(); "
+                        + code], capture_output=True, text=True, timeout=300,
+                       env={**os.environ, "DSMC_SEED": str(seed)})
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def test_pipeline_hands_over_what_the_files_hold(lib, tmp_path):
+    """dsmc_pipeline (main.c:153-204 without the ray tracer) writes the thermal
+    model's T file and both phases' files; each phase's in-memory input must be
+    exactly what the previous phase's files hold: phase 1 equals run_ballistic
+    on the pipeline's T file, and phase 2 equals run_advanced on the pipeline's
+    phase-1 files, per particle byte for byte (density sums to the last
+    printed digit: atomics add in any order)."""
+    from dsmc_suite_amd.capi import SphGrid, ThermalParams
+    a, b, c = (str(tmp_path / x) for x in ("a", "b", "c"))
+    out = run_python(f"lib.dsmc_pipeline.argtypes = [C.c_int, C.c_char_p, C.c_int]; "
+                     f"assert lib.dsmc_pipeline(300, {a!r}.encode(), 4) == 0", seed=6)
+    for line in ("Thermal model done.", "Starting ballistic simulation: 300 particles", "Read ",
+                 "Advanced DSMC complete."):
+        assert line in out, line
+    # the T file: the thermal model as the CLI writes it ("%8.6f", main.c:178-186)
+    tp = ThermalParams()
+    lib.thermal_params_default(C.byref(tp))
+    T = np.zeros(36 * 72)
+    assert lib.dsmc_thermal_solve(36, 72, C.byref(tp), None, ptr(T, C.c_double)) == 0
+    assert (tmp_path / "a_temperature.dat").read_text() == "36\n72\n" + "".join("%8.6f\n" % v for v in T)
+    tfile = f"{a}_temperature.dat"
+    run_python(f"lib.run_ballistic(300, {tfile!r}.encode(), {b!r}.encode(), 4)", seed=6)
+    assert (tmp_path / "a_phase1_ttime.dat").read_bytes() == (tmp_path / "b_ttime.dat").read_bytes()
+    den1, vr1 = f"{a}_phase1_den.dat", f"{a}_phase1_v_record.dat"
+    run_python(f"lib.run_advanced(300, {tfile!r}.encode(), {den1!r}.encode(), {vr1!r}.encode(), "
+               f"{c!r}.encode(), 4)", seed=6)
+    assert (tmp_path / "a_phase2_ttime.dat").read_bytes() == (tmp_path / "c_ttime.dat").read_bytes()
+    for f in ("den", "horizon"):
+        x = np.array((tmp_path / f"a_phase2_{f}.dat").read_text().split(), float)
+        y = np.array((tmp_path / f"c_{f}.dat").read_text().split(), float)
+        assert len(x) == len(y) and np.max(np.abs(x - y)) <= 2e-6, f
+    # phase 2 received phase 1's velocity records (at the reference's defaults the
+    # phase-1 density, a raw weight sum, is too thin for collisions: none happen)
+    nrec = int((tmp_path / "a_phase1_v_record.dat").read_text().split("\n", 1)[0])
+    assert nrec > 0 and f"Read {nrec} velocity records" in out
+    tt = np.array((tmp_path / "a_phase2_ttime.dat").read_text().split(), float).reshape(9, 300)
+    assert np.all(tt[8] == 0) and np.all(tt[4] >= 0)
+
+
 def test_fits_bitwise(lib, dev):
     """Device binning (counting sort) + per-cell reverse-CDF polyfit ==
     advanced.c:178-209 + :32-117 as restated by the oracle."""
