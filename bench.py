@@ -115,6 +115,57 @@ def thermal_cpu(threads: int) -> dict:
     return out
 
 
+def bumpy_mesh(lib, levels: int, noise: float = 0.15, seed: int = 2):
+    """An icosphere (unit radius) with radial noise: a body that shadows itself."""
+    nv, na, nf = C.c_int32(), C.c_int32(), C.c_int32()
+    lib.dsmc_icosphere_sizes(levels, C.byref(nv), C.byref(na), C.byref(nf))
+    vert = np.zeros(nv.value * 3)
+    faces = np.zeros(na.value * 3, np.int32)
+    lib.dsmc_icosphere_build(levels, 0.0, 100.0, 135.0, 3e-26, 1.38e-23, ptr(vert, C.c_double),
+                             ptr(faces, C.c_int32), None, None)
+    vert = vert.reshape(-1, 3) * (1.0 + noise * np.random.default_rng(seed).uniform(-1, 1, (nv.value, 1)))
+    return np.ascontiguousarray(vert), np.ascontiguousarray(faces.reshape(-1, 3)[-nf.value:])
+
+
+SHADOW_CASES = ((5, 24, 50), (3, 24, 50))     # icosphere level, time steps, MC samples (CLI defaults)
+
+
+def shadow_run(lib) -> dict:
+    """The MC shadow flux (shadow_maker.c run_raytracer's computation) on the
+    device for bumpy bodies of 20480 and 1280 faces at the CLI's defaults
+    (za 12.3 deg, 24 time steps, 50 samples per face); wall time of
+    dsmc_shadow_flux (normals, BVH build, upload, kernel, result back)."""
+    out = {}
+    for lev, tt, nm in SHADOW_CASES:
+        v, f = bumpy_mesh(lib, lev)
+        F = np.zeros(len(f) * tt)
+        ms = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            if lib.dsmc_shadow_flux(len(v), ptr(v, C.c_double), len(f), ptr(f, C.c_int32), C.c_double(12.3), tt, nm,
+                                    ptr(F, C.c_double)):
+                raise RuntimeError("dsmc_shadow_flux failed")
+            ms.append((time.perf_counter() - t0) * 1e3)
+        rays = int(np.count_nonzero(F.reshape(len(f), tt) > 0) * nm)
+        out[f"{len(f)}_faces"] = {"ms": round(min(ms), 3), "lit_face_steps": int(np.count_nonzero(F > 0)),
+                                  "rays_per_s": rays / (min(ms) / 1e3)}
+    return out
+
+
+def shadow_cpu(lib, threads: int) -> dict:
+    """The oracle's shadow flux (occlusion tested against every face) on the
+    host cores for the 1280-face body."""
+    from tests.helpers import oracle
+    orc = oracle(devmath=False)
+    lev, tt, nm = SHADOW_CASES[1]
+    v, f = bumpy_mesh(lib, lev)
+    F = np.zeros(len(f) * tt)
+    t0 = time.perf_counter()
+    orc.cpu_ref_shadow_flux(len(v), ptr(v, C.c_double), len(f), ptr(f, C.c_int32), C.c_double(12.3), tt, nm,
+                            ptr(F, C.c_double), threads)
+    return {"cores": threads, f"{len(f)}_faces_ms": round((time.perf_counter() - t0) * 1e3, 3)}
+
+
 def plain_run(lib, surf, n, args, shape) -> dict:
     """Stages one plain run of n particles and times args.steps complete executions."""
     p = capi.preset(lib, PRESET_CERES_PHYSICAL, 36, 72, n=n, seed=args.seed)
@@ -456,6 +507,7 @@ def main():
     ap.add_argument("--no-coll", action="store_true")
     ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in run_ballistic files leg")
     ap.add_argument("--no-thermal", action="store_true", help="skip the thermal model leg")
+    ap.add_argument("--no-shadow", action="store_true", help="skip the MC shadow flux leg")
     ap.add_argument("--coll-particles", type=int, default=2_000_000)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only copy a buffer of this many MiB device to device 5 times (HBM counter calibration)")
@@ -502,6 +554,9 @@ def main():
     if not args.no_thermal:
         progress("thermal model (36x72, 360x720)")
         line["thermal"] = thermal_run(lib)
+    if not args.no_shadow:
+        progress("MC shadow flux (20480 and 1280 faces)")
+        line["shadow"] = shadow_run(lib)
     tuned = None
     if not args.no_batch:
         progress("batched: 64 runs (autotune, launches, plain sequential)")
@@ -535,6 +590,8 @@ def main():
         line["cpu_baseline"] = cpu_baseline(p, surf, args.cpu_sample, threads)
         if not args.no_thermal:
             line["cpu_baseline"]["thermal"] = thermal_cpu(threads)
+        if not args.no_shadow:
+            line["cpu_baseline"]["shadow"] = shadow_cpu(lib, threads)
     print(json.dumps(line))
 
 

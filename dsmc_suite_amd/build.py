@@ -22,7 +22,8 @@ BUILD = ROOT / "_build"
 LIB = PKG / "libdsmc_hip.so"
 ARCH = os.environ.get("DSMC_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels.hip", "host.hip", "advanced.hip", "pool.hip", "batch.hip", "serve.hip", "thermal.hip"]
+HIP_SOURCES = ["kernels.hip", "host.hip", "advanced.hip", "pool.hip", "batch.hip", "serve.hip", "thermal.hip",
+               "shadow.hip"]
 CXX_SOURCES = ["io.cpp", "surface.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-munsafe-fp-atomics",

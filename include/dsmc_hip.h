@@ -416,6 +416,27 @@ void thermal_solve(dsmc_sph_grid *grid, const dsmc_thermal_params *tp, const dou
 /* The same with a status: T_surface [ftd*ffd]; 0 or a dsmc_status. */
 int dsmc_thermal_solve(int ftd, int ffd, const dsmc_thermal_params *tp, const double *flux, double *T_surface);
 
+/* ------------------------------------------------------------------ MC shadow flux */
+/* src/raytracer/shadow_maker.c: for every face nf and time step t, the sun at
+ * longitude (t + 0.5) / tt * 2 pi + pi and elevation -za; F[nf*tt + t] = (1 -
+ * shadowed / Nmonte) * max(0, sun . normal), with Nmonte points drawn on the
+ * face (glibc rand_r seeded t*nface + nf + 42, as the reference) and each
+ * point shadowed when the ray from it (offset 1e-3 along the normal) toward
+ * the sun hits any face.  Normals as mesh_compute_normals (outward = away from
+ * the origin).  The reference tests rays with Embree in single precision; here
+ * the test is double-precision Moller-Trumbore (closed triangles, t > 0) through
+ * a bounding-volume hierarchy, one lane per (face, time step).
+ * vert [npoint*3]; faces [nface*3], 0-based; F [nface*tt].  0 or a dsmc_status. */
+int dsmc_shadow_flux(int npoint, const double *vert, int nface, const int32_t *faces, double za_deg, int tt,
+                     int nmonte, double *F);
+/* glibc rand_r's sequence from `seed` (n values) as the shadow kernel computes
+ * it; no counterpart in the reference (it checks that restatement). */
+int dsmc_rand_r_sequence(unsigned int seed, int n, int *out);
+/* shadow_maker.c:242-271: reads the vertex and (1-based) face files, writes
+ * output_file with write_flux_dat; nthreads is unused (device). */
+void run_raytracer(const char *mesh_vertex_file, const char *mesh_face_file, int npoint, int nface,
+                   double za_deg, int tt, int Nmonte, const char *output_file, int nthreads);
+
 /* ------------------------------------------------------------------ misc */
 const char *dsmc_version(void);
 int dsmc_device_info(char *buf, size_t len);   /* "gfx950, 256 CUs, ..." */

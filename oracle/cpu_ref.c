@@ -38,7 +38,9 @@
  * counterpart: it is pinned by invariants (energy drift, launch histogram
  * against the flux weights, emission moments) - see tests/test_oracle_physical.py.
  * The thermal model (src/thermal/thermal_model.c) is pinned bit for bit against
- * the compiled reference (tests/test_thermal.py, oracle/ref_thermal.c).
+ * the compiled reference (tests/test_thermal.py, oracle/ref_thermal.c).  The MC
+ * shadow flux (src/raytracer/shadow_maker.c) cannot be: the reference needs
+ * Embree, absent here; it is pinned by closed forms (tests/test_shadow.py).
  */
 #include <math.h>
 #include <stdio.h>
@@ -1435,6 +1437,89 @@ int cpu_ref_thermal(int ftd, int ffd, const dsmc_thermal_params *tp, const doubl
     }
     free(a); free(b); free(c); free(F);
     return h;
+}
+
+/* ------------------------------------------------------------------ MC shadow flux */
+/* src/raytracer/shadow_maker.c restated with the occlusion test against every
+ * face (no acceleration structure): double-precision Moller-Trumbore, closed
+ * triangles, hit when t > 0 (the reference's Embree test is single precision).
+ * Sample points from glibc rand_r itself, seeded as the reference. */
+static void sh_normalize(const double v[3], double out[3])
+{
+    double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (l < 1e-30) { out[0] = out[1] = out[2] = 0.0; return; }
+    double s = 1.0 / l;
+    out[0] = v[0] * s; out[1] = v[1] * s; out[2] = v[2] * s;
+}
+
+static int sh_tri_hit(const double o[3], const double d[3], const double *a, const double *b, const double *c)
+{
+    double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    double px = d[1] * e2[2] - d[2] * e2[1], py = d[2] * e2[0] - d[0] * e2[2], pz = d[0] * e2[1] - d[1] * e2[0];
+    double det = e1[0] * px + e1[1] * py + e1[2] * pz;
+    if (det == 0.0) return 0;
+    double sx = o[0] - a[0], sy = o[1] - a[1], sz = o[2] - a[2];
+    double u = (sx * px + sy * py + sz * pz) / det;
+    if (!(u >= 0.0 && u <= 1.0)) return 0;
+    double qx = sy * e1[2] - sz * e1[1], qy = sz * e1[0] - sx * e1[2], qz = sx * e1[1] - sy * e1[0];
+    double v = (d[0] * qx + d[1] * qy + d[2] * qz) / det;
+    if (!(v >= 0.0 && u + v <= 1.0)) return 0;
+    double t = (e2[0] * qx + e2[1] * qy + e2[2] * qz) / det;
+    return t > 0.0;
+}
+
+/* faces 0-based; F[nf*tt + t].  Returns 0 or -1. */
+int cpu_ref_shadow_flux(int npoint, const double *vert, int nface, const int32_t *faces, double za_deg, int tt,
+                        int nmonte, double *F, int nthreads)
+{
+    if (npoint <= 0 || nface <= 0 || tt <= 0 || nmonte <= 0) return -1;
+    double *nrm = (double *)malloc(sizeof(double) * 3 * (size_t)nface);
+    for (int nf = 0; nf < nface; nf++) {                      /* shadow_maker.c:70-110 */
+        const double *a = vert + 3 * faces[3 * nf], *b = vert + 3 * faces[3 * nf + 1], *c = vert + 3 * faces[3 * nf + 2];
+        double cen[3], c1[3], c2[3], c1n[3], c2n[3], c3[3], n[3];
+        for (int k = 0; k < 3; k++) {
+            cen[k] = (a[k] + b[k] + c[k]) / 3.0;
+            c1[k] = b[k] - a[k];
+            c2[k] = c[k] - a[k];
+        }
+        sh_normalize(c2, c2n);
+        sh_normalize(c1, c1n);
+        c3[0] = c2n[1] * c1n[2] - c2n[2] * c1n[1];
+        c3[1] = c2n[2] * c1n[0] - c2n[0] * c1n[2];
+        c3[2] = c2n[0] * c1n[1] - c2n[1] * c1n[0];
+        sh_normalize(c3, n);
+        if (n[0] * cen[0] + n[1] * cen[1] + n[2] * cen[2] < 0)
+            for (int k = 0; k < 3; k++) n[k] = n[k] * -1.0;
+        for (int k = 0; k < 3; k++) nrm[3 * nf + k] = n[k];
+    }
+    const double za = za_deg / 180.0 * M_PI;
+#pragma omp parallel for collapse(2) num_threads(nthreads > 0 ? nthreads : 1) schedule(dynamic)
+    for (int t = 0; t < tt; t++)
+        for (int nf = 0; nf < nface; nf++) {                  /* shadow_maker.c:198-239 */
+            unsigned int seed = (unsigned int)(t * nface + nf + 42);
+            double fi = ((t + 0.5) / tt) * 2.0 * M_PI + M_PI;
+            double sun[3] = {OM_COS(-za) * OM_SIN(fi), OM_COS(-za) * OM_COS(fi), OM_SIN(-za)};
+            const double *n = nrm + 3 * nf;
+            double c6 = sun[0] * n[0] + sun[1] * n[1] + sun[2] * n[2];
+            if (c6 < 0) { F[(size_t)nf * tt + t] = 0.0; continue; }
+            const double *a = vert + 3 * faces[3 * nf], *b = vert + 3 * faces[3 * nf + 1], *c = vert + 3 * faces[3 * nf + 2];
+            int shadow = 0;
+            for (int mc = 0; mc < nmonte; mc++) {
+                double r1 = sqrt((double)rand_r(&seed) / RAND_MAX);
+                double r2 = (double)rand_r(&seed) / RAND_MAX;
+                double o[3];
+                for (int k = 0; k < 3; k++)
+                    o[k] = a[k] * (1.0 - r1) + b[k] * (r1 * (1.0 - r2)) + c[k] * (r1 * r2) + (double)1e-3f * n[k];
+                int hit = 0;
+                for (int g = 0; g < nface && !hit; g++)
+                    hit = sh_tri_hit(o, sun, vert + 3 * faces[3 * g], vert + 3 * faces[3 * g + 1],
+                                     vert + 3 * faces[3 * g + 2]);
+                shadow += hit;
+            }
+            F[(size_t)nf * tt + t] = (1.0 - (double)shadow / nmonte) * c6;
+        }
+    free(nrm);
+    return 0;
 }
 
 int cpu_ref_max_threads(void) { return omp_get_max_threads(); }
