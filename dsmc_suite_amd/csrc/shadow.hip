@@ -3,10 +3,12 @@
  * shadow_maker.c: mesh_load, mesh_compute_normals, random_point_on_triangle,
  * flux_compute_embree, run_raytracer) on the device.
  *
- * One lane per (face, time step): the sun direction of the step, the face's
- * cosine factor, then Nmonte points drawn on the face with the reference's
- * glibc rand_r sequence (seed t*nface + nf + 42) and, for each, an any-hit
- * occlusion ray toward the sun.  The reference queries Embree's single-
+ * One wavefront per (face, time step): the sun direction of the step, the face's
+ * cosine factor, then the Nmonte points the reference draws on the face with
+ * glibc rand_r (seed t*nface + nf + 42), one per lane (each lane jumps to its
+ * point's place in the sequence: rand_r is three steps of a linear
+ * congruential generator, and k steps of it are one affine map), an any-hit
+ * occlusion ray toward the sun from each, and the wavefront's count of hits.  The reference queries Embree's single-
  * precision BVH; here the test is double-precision Moller-Trumbore (closed
  * triangles, hit when t > 0) and the acceleration structure a binary BVH built
  * on the host (median split of face centroids along the longest axis, at most
@@ -73,10 +75,36 @@ __host__ __device__ static inline int rand_r_glibc(unsigned int *seed)
     return result;
 }
 
+/* The generator state k steps of x -> 1103515245 x + 12345 (mod 2^32) after s:
+ * powers of one affine map, composed by repeated squaring. */
+__host__ __device__ static inline unsigned int lcg_skip(unsigned int s, unsigned int k)
+{
+    unsigned int A = 1u, C = 0u, a = 1103515245u, c = 12345u;
+    while (k) {
+        if (k & 1u) { A = a * A; C = a * C + c; }
+        c = a * c + c;
+        a = a * a;
+        k >>= 1;
+    }
+    return A * s + C;
+}
+
+/* n values of the sequence: in turn (dsmc_rand_r_sequence), or each reached by
+ * jumping 3 i generator steps from `seed`, the kernel's way (dsmc_rand_r_at). */
 extern "C" int dsmc_rand_r_sequence(unsigned int seed, int n, int *out)
 {
     if (n < 0 || (n && !out)) return DSMC_ERR_ARG;
     for (int i = 0; i < n; i++) out[i] = rand_r_glibc(&seed);
+    return DSMC_OK;
+}
+
+extern "C" int dsmc_rand_r_at(unsigned int seed, int n, int *out)
+{
+    if (n < 0 || (n && !out)) return DSMC_ERR_ARG;
+    for (int i = 0; i < n; i++) {
+        unsigned int s = lcg_skip(seed, 3u * (unsigned int)i);
+        out[i] = rand_r_glibc(&s);
+    }
     return DSMC_OK;
 }
 
@@ -98,8 +126,9 @@ __device__ static inline bool tri_hit(const double o[3], const double d[3], cons
     return t > 0.0;
 }
 
-/* The ray meets the box at some t >= 0. */
-__device__ static inline bool box_hit(const double o[3], const double d[3], const BvhNode &n)
+/* The ray meets the box at some t >= 0 (id = 1 / d; the boxes are padded, so
+ * the rounding of the products cannot reject a face the triangle test accepts). */
+__device__ static inline bool box_hit(const double o[3], const double d[3], const double id[3], const BvhNode &n)
 {
     double t0 = 0.0, t1 = INFINITY;
     for (int k = 0; k < 3; k++) {
@@ -107,7 +136,7 @@ __device__ static inline bool box_hit(const double o[3], const double d[3], cons
             if (o[k] < n.lo[k] || o[k] > n.hi[k]) return false;
             continue;
         }
-        double a = (n.lo[k] - o[k]) / d[k], b = (n.hi[k] - o[k]) / d[k];
+        double a = (n.lo[k] - o[k]) * id[k], b = (n.hi[k] - o[k]) * id[k];
         if (a > b) { double s = a; a = b; b = s; }
         t0 = a > t0 ? a : t0;
         t1 = b < t1 ? b : t1;
@@ -116,14 +145,15 @@ __device__ static inline bool box_hit(const double o[3], const double d[3], cons
     return true;
 }
 
-__device__ static bool occluded(const double o[3], const double d[3], const BvhNode *nodes, const double *tris)
+__device__ static bool occluded(const double o[3], const double d[3], const double id[3], const BvhNode *nodes,
+                                const double *tris)
 {
     int stack[BVH_STACK];
     int sp = 0;
     stack[sp++] = 0;
     while (sp > 0) {
         const BvhNode n = nodes[stack[--sp]];
-        if (!box_hit(o, d, n)) continue;
+        if (!box_hit(o, d, id, n)) continue;
         if (n.count > 0) {
             for (int i = 0; i < n.count; i++)
                 if (tri_hit(o, d, tris + (size_t)(n.first + i) * 9)) return true;
@@ -135,31 +165,43 @@ __device__ static bool occluded(const double o[3], const double d[3], const BvhN
     return false;
 }
 
-/* face: [nface][12] = a, b, c, normal; F[nf * tt + t] (shadow_maker.c:198-239) */
+/* face: [nface][12] = a, b, c, normal; F[nf * tt + t] (shadow_maker.c:198-239).
+ * One wavefront per (face, step); blocks of whole wavefronts. */
 __global__ void __launch_bounds__(256) shadow_kernel(int nface, int tt, int nmonte, double za, const double *face,
                                                      const BvhNode *nodes, const double *tris, double *F)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;     /* uniform in the wavefront */
+    const int lane = (int)(threadIdx.x & 63);
     if (i >= (int64_t)nface * tt) return;
     const int nf = (int)(i / tt), t = (int)(i % tt);
     const double fi = ((t + 0.5) / tt) * 2.0 * M_PI + M_PI;
     const double cz = dm_cos(-za);
     const double sun[3] = {cz * dm_sin(fi), cz * dm_cos(fi), dm_sin(-za)};
+    const double isun[3] = {1.0 / sun[0], 1.0 / sun[1], 1.0 / sun[2]};
     const double *f = face + (size_t)nf * 12;
     const double c6 = sun[0] * f[9] + sun[1] * f[10] + sun[2] * f[11];
-    if (c6 < 0) { F[i] = 0.0; return; }
-    unsigned int seed = (unsigned int)(t * nface + nf + 42);
+    if (c6 < 0) {
+        if (lane == 0) F[i] = 0.0;
+        return;
+    }
+    const unsigned int seed0 = (unsigned int)(t * nface + nf + 42);
     const double eps = (double)1e-3f;
     int shadow = 0;
-    for (int mc = 0; mc < nmonte; mc++) {
-        const double r1 = sqrt((double)rand_r_glibc(&seed) / 2147483647);
-        const double r2 = (double)rand_r_glibc(&seed) / 2147483647;
-        const double wa = 1.0 - r1, wb = r1 * (1.0 - r2), wc = r1 * r2;
-        double o[3];
-        for (int k = 0; k < 3; k++) o[k] = f[k] * wa + f[3 + k] * wb + f[6 + k] * wc + eps * f[9 + k];
-        if (occluded(o, sun, nodes, tris)) shadow++;
+    for (int base = 0; base < nmonte; base += 64) {
+        const int mc = base + lane;
+        bool hit = false;
+        if (mc < nmonte) {
+            unsigned int seed = lcg_skip(seed0, 6u * (unsigned int)mc);   /* two rand_r calls per point */
+            const double r1 = sqrt((double)rand_r_glibc(&seed) / 2147483647);
+            const double r2 = (double)rand_r_glibc(&seed) / 2147483647;
+            const double wa = 1.0 - r1, wb = r1 * (1.0 - r2), wc = r1 * r2;
+            double o[3];
+            for (int k = 0; k < 3; k++) o[k] = f[k] * wa + f[3 + k] * wb + f[6 + k] * wc + eps * f[9 + k];
+            hit = occluded(o, sun, isun, nodes, tris);
+        }
+        shadow += __popcll(__ballot(hit));
     }
-    F[i] = (1.0 - (double)shadow / nmonte) * c6;
+    if (lane == 0) F[i] = (1.0 - (double)shadow / nmonte) * c6;
 }
 
 /* ------------------------------------------------------------------ host */
@@ -292,8 +334,9 @@ static int shadow_run(int npoint, const double *vert, int nface, const int32_t *
     HIP_OK(hipMemcpy(d_face, face.data(), face.size() * 8, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(d_tris, tris.data(), tris.size() * 8, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(d_nodes, B.nodes.data(), B.nodes.size() * sizeof(BvhNode), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(shadow_kernel, dim3((unsigned)((nF + 255) / 256)), dim3(256), 0, 0, nface, tt, nmonte, za,
-                       d_face, d_nodes, d_tris, d_F);
+    /* four (face, step) wavefronts per 256-thread block */
+    hipLaunchKernelGGL(shadow_kernel, dim3((unsigned)((nF + 3) / 4)), dim3(256), 0, 0, nface, tt, nmonte, za, d_face,
+                       d_nodes, d_tris, d_F);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpy(F, d_F, nF * 8, hipMemcpyDeviceToHost));
 done:

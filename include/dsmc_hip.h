@@ -432,6 +432,9 @@ int dsmc_shadow_flux(int npoint, const double *vert, int nface, const int32_t *f
 /* glibc rand_r's sequence from `seed` (n values) as the shadow kernel computes
  * it; no counterpart in the reference (it checks that restatement). */
 int dsmc_rand_r_sequence(unsigned int seed, int n, int *out);
+/* The same n values, each reached by jumping 3 i generator steps from `seed`
+ * (how the kernel's lanes find their sample points). */
+int dsmc_rand_r_at(unsigned int seed, int n, int *out);
 /* shadow_maker.c:242-271: reads the vertex and (1-based) face files, writes
  * output_file with write_flux_dat; nthreads is unused (device). */
 void run_raytracer(const char *mesh_vertex_file, const char *mesh_face_file, int npoint, int nface,

@@ -66,6 +66,9 @@ def test_rand_r_restatement_equals_glibc():
         s = C.c_uint(seed)
         want = [libc.rand_r(C.byref(s)) for _ in range(5000)]
         assert np.array_equal(got, want), seed
+        jumped = np.zeros(5000, np.int32)                 # each value by skipping ahead
+        assert lib.dsmc_rand_r_at(C.c_uint(seed), 5000, jumped.ctypes.data_as(IP)) == 0
+        assert np.array_equal(jumped, want), seed
 
 
 def test_convex_body_casts_no_shadow():
@@ -113,7 +116,7 @@ def test_gpu_shadow_flux_bitwise(tmp_path):
     vb, fb = sphere(3, noise=0.15, seed=2)
     vp, fp = with_plate(*sphere(3))
     for name, v, f, za, tt, nm in (("bumpy", vb, fb, 12.3, 6, 16), ("plate", vp, fp, -90.0, 3, 12),
-                                   ("bumpy low sun", vb, fb, 70.0, 4, 8)):
+                                   ("bumpy low sun", vb, fb, 70.0, 4, 8), ("bumpy, 150 samples", vb, fb, 30.0, 2, 150)):
         print(f"shadow kernel: {name}", flush=True)
         got = np.zeros(len(f) * tt)
         assert lib.dsmc_shadow_flux(len(v), v.ctypes.data_as(DP), len(f), f.ctypes.data_as(IP), za, tt, nm,
