@@ -339,14 +339,22 @@ dsmc_launch_shape default_shape(const DsmcEnv &E, int short_hops)
         s.relaunch_batch = 48;
         s.step_budget = 2048;
         s.service_inline = 1;
+    } else if (E.nparticles < 4000000) {
+        /* long hops (Ceres), runs below 4e6 particles: the landing service inline
+         * in batches of 24 lanes, one launch (3e6 particles 82 ms against 87 ms
+         * with the split kernels, 1e6 44 against 47, 3e4 21 against 24) */
+        s.blocks_per_cu = 4;
+        s.relaunch_batch = 24;
+        s.step_budget = INT_MAX;
+        s.service_inline = 1;
     } else {
-        /* long hops (Ceres): the lean flight kernel.  1e7 particles: 5 blocks per
-         * CU and a 4096-step budget in the first two rounds (1024 after them)
-         * 1.31e11 particle-steps/s (4 blocks, 1024 steps throughout: 1.13e11);
-         * runs of 1e4-2e6 particles do best with 1024 steps throughout */
+        /* long hops (Ceres), 4e6 particles and more: the lean flight kernel.  1e7
+         * particles: 5 blocks per CU and a 4096-step budget in the first two
+         * rounds (1024 after them) 1.31e11 particle-steps/s (4 blocks, 1024 steps
+         * throughout: 1.13e11; the service inline: 203 ms per run, a tie) */
         s.blocks_per_cu = 5;
         s.relaunch_batch = 16;
-        s.step_budget = E.nparticles >= 4000000 ? 4096 : 1024;
+        s.step_budget = 4096;
         s.service_inline = 0;
     }
     return s;
