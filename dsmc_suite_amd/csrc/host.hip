@@ -390,10 +390,12 @@ int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st
         if (rd->ttime.ptr)
             HIP_OK(hipMemcpyAsync(r->ttime, rd->ttime.ptr, rd->ttime.bytes, hipMemcpyDeviceToHost, st));
     }
-    if (r->end_state) HIP_OK(hipMemcpyAsync(r->end_state, rd->end_state.ptr, 48 * n, hipMemcpyDeviceToHost, st));
-    if (r->steps) HIP_OK(hipMemcpyAsync(r->steps, rd->steps.ptr, 8 * n, hipMemcpyDeviceToHost, st));
-    if (r->fate) HIP_OK(hipMemcpyAsync(r->fate, rd->fate.ptr, 4 * n, hipMemcpyDeviceToHost, st));
-    if (r->hops) HIP_OK(hipMemcpyAsync(r->hops, rd->hops.ptr, 4 * n, hipMemcpyDeviceToHost, st));
+    /* per-particle outputs (a run of no particles has none: no device buffer) */
+    if (r->end_state && rd->end_state.ptr)
+        HIP_OK(hipMemcpyAsync(r->end_state, rd->end_state.ptr, 48 * n, hipMemcpyDeviceToHost, st));
+    if (r->steps && rd->steps.ptr) HIP_OK(hipMemcpyAsync(r->steps, rd->steps.ptr, 8 * n, hipMemcpyDeviceToHost, st));
+    if (r->fate && rd->fate.ptr) HIP_OK(hipMemcpyAsync(r->fate, rd->fate.ptr, 4 * n, hipMemcpyDeviceToHost, st));
+    if (r->hops && rd->hops.ptr) HIP_OK(hipMemcpyAsync(r->hops, rd->hops.ptr, 4 * n, hipMemcpyDeviceToHost, st));
     if (r->vhist && rd->vhist.ptr)
         HIP_OK(hipMemcpyAsync(r->vhist, rd->vhist.ptr, rd->vhist.bytes, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
@@ -415,6 +417,8 @@ int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st
         }
         r->vrec_n = nrec;
     }
+    /* physical runs have no transit times: their ttime buffer (5 rows, ballistic
+     * or advanced) is zeroed; a compat run has a device buffer unless n == 0 */
     if (r->ttime && !rd->ttime.ptr) memset(r->ttime, 0, sizeof(double) * 5 * n);
     r->n_escaped = (int64_t)ctl[0];
     r->n_frozen = (int64_t)ctl[1];

@@ -169,7 +169,7 @@ typedef struct dsmc_result {
     double *horizon;          /* [ftd*ffd] frozen / polar deposits */
     double *vmom;             /* [hd*ftd*ffd*3] sum of velocity per deposit (physical) */
     int64_t *vcount;          /* [hd*ftd*ffd] deposits per cell (physical) */
-    double *ttime;            /* [rows*n], rows 5 (ballistic) or 9 (advanced) */
+    double *ttime;            /* [rows*n], rows 5 (ballistic) or 9 (advanced); compat runs only: a physical run zeroes 5 rows */
     double *end_state;        /* [n*6] final position and velocity */
     int64_t *steps;           /* [n] Verlet steps taken by each particle */
     int32_t *fate;            /* [n] enum dsmc_fate */

@@ -148,6 +148,53 @@ def test_physical_bitwise(lib, dev):
     assert np.allclose(got.vmom, want.vmom, rtol=1e-9, atol=1e-6)
 
 
+def check_particles(got, want):
+    for k in ("steps", "end_state", "fate", "hops"):
+        assert np.array_equal(getattr(got, k), getattr(want, k)), k
+    assert got.counters() == want.counters()
+    assert np.allclose(got.density, want.density, rtol=1e-12, atol=1e-12)
+    assert np.allclose(got.horizon, want.horizon, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("n", [0, 1, 257])
+def test_edge_particle_counts(lib, dev, n):
+    """A run of no particle, of one, and of one past a full 256-lane block:
+    compat ballistic through the plain API, physical Ceres through the plain
+    API and through both kernel paths with explicit shapes (split kernels,
+    service inline with 37-step launches): bit for bit as the oracle; a run of
+    no particle leaves every output zero."""
+    T, ftd, ffd = read_T(GOLDEN / "T_thermal.dat")
+    print(f"edge counts: compat ballistic, {n} particles", flush=True)
+    p = compat_params(ftd, ffd, n, 23)
+    surf = SurfaceArrays(T, ftd, ffd)
+    got = Buffers(p, 5)
+    got.ttime[:] = np.nan
+    assert lib.dsmc_run_ballistic(C.byref(p), C.byref(surf.surface), C.byref(got.result)) == 0
+    want = run_oracle(dev, p, surf, nthreads=8)
+    check_particles(got, want)
+    assert np.array_equal(got.ttime, want.ttime)
+    pp = capi.preset(lib, PRESET_CERES_PHYSICAL, 36, 72, n=n, seed=5)
+    sp = icosphere(lib, 3)
+    want = run_oracle(dev, pp, sp, nthreads=8)
+    for shape in (None, (5, 16, 4096, 0), (4, 24, 37, 1)):
+        print(f"edge counts: physical Ceres, {n} particles, shape {shape}", flush=True)
+        got = Buffers(pp, 5)
+        if shape is None:
+            assert lib.dsmc_run_ballistic(C.byref(pp), C.byref(sp.surface), C.byref(got.result)) == 0
+        else:
+            st = C.c_int(0)
+            run = lib.dsmc_run_create(C.byref(pp), C.byref(sp.surface), C.byref(got.result), C.byref(st))
+            assert run and st.value == 0
+            sh = capi.LaunchShape(256, *shape)
+            assert lib.dsmc_run_launch(run, C.byref(sh)) == 0 and lib.dsmc_run_sync(run) == 0
+            assert lib.dsmc_run_collect(run, C.byref(got.result)) == 0
+            lib.dsmc_run_destroy(run)
+        check_particles(got, want)
+    if n == 0:
+        assert not got.density.any() and not got.horizon.any()
+        assert all(v == 0 for v in got.counters().values())
+
+
 def test_velocity_records_caps(lib):
     """First-come velocity records (ballistic.c:215-232): at most 200 per
     (td, ht), total within the cap, and every record's cell indices valid."""
