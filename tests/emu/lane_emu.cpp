@@ -292,3 +292,29 @@ extern "C" int emu_locate(const dsmc_params *p, const dsmc_surface *s, int64_t n
     }
     return 0;
 }
+
+/* Which step of locate_face_near resolves each of n directions (no hint):
+ * counts[0] the first four face-map candidates, [1] the next four, [2] the
+ * hierarchical search (no candidate holds the direction firmly). */
+extern "C" int emu_locate_paths(const dsmc_params *p, const dsmc_surface *s, int64_t n, const double *dirs,
+                                int64_t *counts)
+{
+    if (p->mode != DSMC_MODE_PHYSICAL) return -1;
+    DsmcEnv E;
+    dsmc::HostTables T;
+    int rc = dsmc::build_env(p, s, 0, &E, &T);
+    if (rc) return rc;
+    E.fnorm = T.fnorm.data();
+    attach_face_map(E, T);
+    counts[0] = counts[1] = counts[2] = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const double *d = dirs + 3 * i;
+        const double eps = 1e-12 * sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const int cell = fmap_cell(E.fmap_g, d[0], d[1], d[2]);
+        int found = -1;
+        for (int j = 0; cell >= 0 && j < DSMC_FMAP_K && E.fmap[(size_t)cell * DSMC_FMAP_K + j] >= 0; j++)
+            if (face_holds(E, E.fmap[(size_t)cell * DSMC_FMAP_K + j], d[0], d[1], d[2], eps)) { found = j; break; }
+        counts[found < 0 ? 2 : (found < 4 ? 0 : 1)]++;
+    }
+    return 0;
+}

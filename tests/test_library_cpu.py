@@ -169,6 +169,22 @@ def test_face_map_locates_as_the_full_search(lib, emu, levels):
     assert len(np.unique(full)) == nfine
 
 
+@pytest.mark.parametrize("levels", [3, 4])
+def test_face_map_candidates_cover_the_sphere(lib, emu, levels):
+    """The face map's first four candidates hold nearly every direction firmly, so
+    the landing service's face search rarely needs the second group and almost
+    never the hierarchical search (one lane that does holds up the wavefront)."""
+    vert, faces, fT, fw = build_icosphere(lib, levels)
+    surf = SurfaceArrays(levels=levels, vert=vert.ravel(), faces=faces.ravel(), face_T=fT, face_w=fw)
+    p = capi.preset(lib, PRESET_CERES_PHYSICAL, 36, 72, n=1, seed=1)
+    d = np.ascontiguousarray(np.random.default_rng(7).normal(size=(200000, 3)))
+    cnt = np.zeros(3, np.int64)
+    emu.emu_locate_paths.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+    assert emu.emu_locate_paths(C.byref(p), C.byref(surf.surface), len(d), d.ctypes.data, cnt.ctypes.data) == 0
+    first, second, search = cnt / len(d)
+    assert first > 0.98 and second < 0.02 and search < 1e-3, (first, second, search)
+
+
 @pytest.mark.parametrize("tname", ["T_thermal", "T_polar"])
 def test_lane_state_machine_matches_oracle_compat(emu, tname):
     dev = oracle(devmath=True)
