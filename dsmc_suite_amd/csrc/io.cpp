@@ -662,6 +662,38 @@ static double through_text(double v, int prec)
     return fast_strtod(t.c_str(), &end);
 }
 
+extern "C" void grid_free(dsmc_sph_grid *g)
+{
+    if (!g) return;
+    free(g->density);
+    free(g->temperature);
+    free(g->volume);
+    free(g->horizon);
+    free(g);
+}
+
+extern "C" dsmc_sph_grid *grid_create(int ftd, int ffd, int hd)
+{
+    if (ftd < 0 || ffd < 0 || hd < 0) return nullptr;
+    if (hd > 0 && (size_t)ftd * ffd > SIZE_MAX / sizeof(double) / (size_t)hd) return nullptr;
+    dsmc_sph_grid *g = (dsmc_sph_grid *)calloc(1, sizeof(dsmc_sph_grid));
+    if (!g) return nullptr;
+    g->ftd = ftd;
+    g->ffd = ffd;
+    g->hd = hd;
+    const size_t surf = (size_t)ftd * ffd, cells = surf * hd;
+    /* calloc(0) may return NULL: ask for at least one element */
+    g->density = (double *)calloc(cells ? cells : 1, sizeof(double));
+    g->temperature = (double *)calloc(surf ? surf : 1, sizeof(double));
+    g->volume = (double *)calloc(cells ? cells : 1, sizeof(double));
+    g->horizon = (double *)calloc(surf ? surf : 1, sizeof(double));
+    if (!g->density || !g->temperature || !g->volume || !g->horizon) {
+        grid_free(g);
+        return nullptr;
+    }
+    return g;
+}
+
 extern "C" int dsmc_pipeline(int totaln, const char *output_prefix, int nthreads)
 {
     if (!output_prefix) return DSMC_ERR_ARG;

@@ -407,6 +407,13 @@ typedef struct {
     double *density, *temperature, *volume, *horizon;
 } dsmc_sph_grid;
 
+/* grid.c:7-17: a grid with every array zeroed (density and volume [hd*ftd*ffd],
+ * temperature and horizon [ftd*ffd]); NULL for a negative size or when out of
+ * memory.  The reference CLI allocates the thermal model's grid with it. */
+dsmc_sph_grid *grid_create(int ftd, int ffd, int hd);
+/* grid.c:19-26: frees the grid and its arrays; NULL is ignored. */
+void grid_free(dsmc_sph_grid *g);
+
 /* thermal_model.c:23-49 */
 void thermal_params_default(dsmc_thermal_params *tp);
 /* thermal_model.c:105-283: fills grid->temperature [ftd*ffd]; flux [ftd*ffd]

@@ -310,6 +310,47 @@ def test_pipeline_hands_over_what_the_files_hold(lib, tmp_path):
     assert np.all(tt[8] == 0) and np.all(tt[4] >= 0)
 
 
+REF_CLI = capi.PKG_DIR.parent / "oracle" / "_ref" / "dsmc_cli"
+
+
+def vrec_cells(path):
+    """(record count, records per (td, ht)) of a v_record file (fileio.c:125-141)."""
+    tok = path.read_text().split()
+    n = int(tok[0])
+    rec = np.array(tok[1:], float).reshape(n, 6)
+    cells, counts = np.unique(rec[:, [3, 5]].astype(int), axis=0, return_counts=True)
+    return n, {tuple(c): k for c, k in zip(cells, counts)}
+
+
+def test_reference_cli_pipeline_on_the_library(lib, tmp_path):
+    """The reference CLI (src/main.c, unmodified) linked against this library
+    alone, run as `pipeline -n 300 -t 4` (main.c:153-204: thermal model, T file,
+    run_ballistic, run_advanced on the phase-1 files), against dsmc_pipeline:
+    the T file and both phases' ttime files byte for byte, densities and
+    horizons to the last printed digit (atomics add in any order), the same
+    number of velocity records in every (td, ht) cell (which records fill a
+    capped cell is first come, first kept)."""
+    if not REF_CLI.exists():
+        pytest.skip("the reference CLI (make -C oracle ref) is not built")
+    r, o = str(tmp_path / "r"), str(tmp_path / "o")
+    print("reference CLI: pipeline, 300 particles", flush=True)
+    cli = subprocess.run([str(REF_CLI), "pipeline", "-n", "300", "-t", "4", "-o", r], capture_output=True,
+                         text=True, timeout=300, env={**os.environ, "DSMC_SEED": "6"})
+    assert cli.returncode == 0, cli.stderr
+    assert "=== PIPELINE COMPLETE ===" in cli.stdout
+    run_python(f"lib.dsmc_pipeline.argtypes = [C.c_int, C.c_char_p, C.c_int]; "
+               f"assert lib.dsmc_pipeline(300, {o!r}.encode(), 4) == 0", seed=6)
+    for f in ("temperature", "phase1_ttime", "phase2_ttime"):
+        assert (tmp_path / f"r_{f}.dat").read_bytes() == (tmp_path / f"o_{f}.dat").read_bytes(), f
+    for f in ("phase1_den", "phase1_horizon", "phase2_den", "phase2_horizon"):
+        x = np.array((tmp_path / f"r_{f}.dat").read_text().split(), float)
+        y = np.array((tmp_path / f"o_{f}.dat").read_text().split(), float)
+        assert len(x) == len(y) and np.max(np.abs(x - y)) <= 2e-6, f
+    nr, cr = vrec_cells(tmp_path / "r_phase1_v_record.dat")
+    no, co = vrec_cells(tmp_path / "o_phase1_v_record.dat")
+    assert nr == no and nr > 0 and cr == co
+
+
 def test_fits_bitwise(lib, dev):
     """Device binning (counting sort) + per-cell reverse-CDF polyfit ==
     advanced.c:178-209 + :32-117 as restated by the oracle."""
