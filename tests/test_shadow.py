@@ -18,6 +18,8 @@ for bit:
 from __future__ import annotations
 
 import ctypes as C
+import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -136,3 +138,25 @@ def test_gpu_shadow_flux_bitwise(tmp_path):
     lib.run_raytracer(str(vf).encode(), str(ff).encode(), len(vb), len(fb), 12.3, 6, 16, str(out).encode(), 4)
     want = oracle_flux(dev, vb, fb, 12.3, 6, 16).ravel()
     assert out.read_text() == "%d\n%d\n" % (len(fb), 6) + "".join("%9.7f\n" % x for x in want)
+
+
+@pytest.mark.gpu
+def test_reference_cli_raytrace_on_the_library(tmp_path):
+    """The reference CLI (src/main.c, unmodified, linked against this library
+    alone) as `raytrace -m v -f f --npoint --nface --za --tt --nmonte`
+    (main.c:107-116): its flux file equals the oracle's device-math build."""
+    cli = capi.PKG_DIR.parent / "oracle" / "_ref" / "dsmc_cli"
+    if not cli.exists():
+        pytest.skip("the reference CLI (make -C oracle ref) is not built")
+    dev = oracle(devmath=True)
+    vb, fb = sphere(3, noise=0.15, seed=4)
+    vf, ff = tmp_path / "v.txt", tmp_path / "f.txt"
+    vf.write_text("".join("%.17g %.17g %.17g\n" % tuple(x) for x in vb))
+    ff.write_text("".join("%d %d %d\n" % tuple(x + 1) for x in fb))
+    print("reference CLI: raytrace", flush=True)
+    r = subprocess.run([str(cli), "raytrace", "-m", str(vf), "-f", str(ff), "--npoint", str(len(vb)),
+                        "--nface", str(len(fb)), "--za", "30", "--tt", "5", "--nmonte", "24", "-t", "4",
+                        "-o", str(tmp_path / "r")], capture_output=True, text=True, timeout=300, env=dict(os.environ))
+    assert r.returncode == 0, r.stderr
+    want = oracle_flux(dev, vb, fb, 30.0, 5, 24).ravel()
+    assert (tmp_path / "r_flux.dat").read_text() == "%d\n%d\n" % (len(fb), 5) + "".join("%9.7f\n" % x for x in want)
