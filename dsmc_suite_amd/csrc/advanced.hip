@@ -395,14 +395,13 @@ extern "C" int dsmc_run_advanced(const dsmc_params *p, const dsmc_surface *s, co
     int rc = check_phase2(p, in);
     if (rc || !s || !r || p->nparticles < 0) return rc ? rc : DSMC_ERR_ARG;
     if ((rc = ensure_device())) return rc;
-    hipStream_t st;
-    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    TimedStream ts;
+    if (ts.create()) return DSMC_ERR_DEVICE;
+    hipStream_t st = ts.st;
+    hipEvent_t e0 = ts.e0, e1 = ts.e1;
     RunDev rd;
     rc = stage_run(p, s, r, 1, &rd, st);
     if (!rc) rc = p->mode == DSMC_MODE_PHYSICAL ? stage_phase2_phys(p, in, nullptr, &rd, st) : stage_phase2(p, in, &rd, st);
-    hipEvent_t e0, e1;
-    HIP_OK(hipEventCreate(&e0));
-    HIP_OK(hipEventCreate(&e1));
     if (!rc) {
         HIP_OK(hipEventRecord(e0, st));
         rc = launch_run(&rd, nullptr, st);
@@ -414,9 +413,6 @@ extern "C" int dsmc_run_advanced(const dsmc_params *p, const dsmc_surface *s, co
         HIP_OK(hipEventElapsedTime(&ms, e0, e1));
         r->elapsed_ms = ms;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(st);
     return rc;
 }
 
@@ -496,8 +492,10 @@ extern "C" int dsmc_run_collisional(const dsmc_params *p1, const dsmc_params *p2
         return DSMC_ERR_ARG;
     if ((rc = ensure_device())) return rc;
     const size_t ncell = (size_t)p1->ftd * p1->ffd * p1->hd;
-    hipStream_t st;
-    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    TimedStream ts;
+    if (ts.create()) return DSMC_ERR_DEVICE;
+    hipStream_t st = ts.st;
+    hipEvent_t e0 = ts.e0, e1 = ts.e1;
     /* phase 1: the caller's outputs plus counts and histograms */
     dsmc_result spec1;
     if (r1) spec1 = *r1;
@@ -506,9 +504,6 @@ extern "C" int dsmc_run_collisional(const dsmc_params *p1, const dsmc_params *p2
     spec1.vcount = cnt.data();
     int64_t dummy = 0;
     if (!spec1.vhist) spec1.vhist = &dummy;            /* non-NULL: keep histograms */
-    hipEvent_t e0, e1;
-    HIP_OK(hipEventCreate(&e0));
-    HIP_OK(hipEventCreate(&e1));
     {
         RunDev rd1;
         rc = stage_run(p1, s, &spec1, 0, &rd1, st);
@@ -543,8 +538,5 @@ extern "C" int dsmc_run_collisional(const dsmc_params *p1, const dsmc_params *p2
         HIP_OK(hipEventElapsedTime(&ms, e0, e1));
         r2->elapsed_ms = ms;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(st);
     return rc;
 }

@@ -24,6 +24,29 @@ struct DevBuf {
     int zeros(size_t n, hipStream_t s);
 };
 
+/* A non-blocking stream and two timing events for one call, released on every
+ * return path (declare it before the call's RunDev, so the buffers go first). */
+struct TimedStream {
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TimedStream() = default;
+    TimedStream(const TimedStream &) = delete;
+    TimedStream &operator=(const TimedStream &) = delete;
+    int create()
+    {
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) { st = nullptr; return DSMC_ERR_DEVICE; }
+        if (hipEventCreate(&e0) != hipSuccess) { e0 = nullptr; return DSMC_ERR_DEVICE; }
+        if (hipEventCreate(&e1) != hipSuccess) { e1 = nullptr; return DSMC_ERR_DEVICE; }
+        return DSMC_OK;
+    }
+    ~TimedStream()
+    {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
 /* Device-side state of one run. */
 struct RunDev {
     DsmcEnv E;

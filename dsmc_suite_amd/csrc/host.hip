@@ -266,15 +266,20 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
      * 8192-20000 in the later rounds) */
     const int64_t sh_budget = sh.step_budget;
     const int64_t late_budget = !shape && !inline_svc && sh_budget > 1024 ? 1024 : sh_budget;
-    std::vector<hipEvent_t> ev;
+    /* per-launch timing events, destroyed on every return path */
+    struct Events {
+        std::vector<hipEvent_t> v;
+        ~Events() { for (hipEvent_t e : v) (void)hipEventDestroy(e); }
+    } evs;
+    std::vector<hipEvent_t> &ev = evs.v;
     int p = 0, chunk = 4, issued = 0;
     for (;;) {
         for (int r = 0; r < chunk; r++) {
             if (timed) {
                 hipEvent_t a, b;
                 HIP_OK(hipEventCreate(&a));
-                HIP_OK(hipEventCreate(&b));
                 ev.push_back(a);
+                HIP_OK(hipEventCreate(&b));
                 ev.push_back(b);
                 HIP_OK(hipEventRecord(a, st));
             }
@@ -306,7 +311,6 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
         HIP_OK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
         rd->flight_ms += ms;
     }
-    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     return DSMC_OK;
 }
 
@@ -531,13 +535,12 @@ extern "C" int dsmc_run_ballistic(const dsmc_params *p, const dsmc_surface *s, d
     if (!p || !s || !r || p->nparticles < 0) return DSMC_ERR_ARG;
     int dev_rc = ensure_device();
     if (dev_rc) return dev_rc;
-    hipStream_t st;
-    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    TimedStream ts;
+    if (ts.create()) return DSMC_ERR_DEVICE;
+    hipStream_t st = ts.st;
+    hipEvent_t e0 = ts.e0, e1 = ts.e1;
     RunDev rd;
     int rc = stage_run(p, s, r, 0, &rd, st);
-    hipEvent_t e0, e1;
-    HIP_OK(hipEventCreate(&e0));
-    HIP_OK(hipEventCreate(&e1));
     if (!rc) {
         HIP_OK(hipEventRecord(e0, st));
         rc = launch_run(&rd, nullptr, st);
@@ -549,9 +552,6 @@ extern "C" int dsmc_run_ballistic(const dsmc_params *p, const dsmc_surface *s, d
         HIP_OK(hipEventElapsedTime(&ms, e0, e1));
         r->elapsed_ms = ms;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(st);
     return rc;
 }
 
