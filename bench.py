@@ -19,6 +19,7 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -80,13 +81,27 @@ def cpu_baseline(p, surf, sample: int, threads: int) -> dict:
 THERMAL_GRIDS = ((36, 72), (360, 720))
 
 
+def thermal_depth_points(tp, ffd: int) -> int:
+    """Depth grid of the thermal model (thermal_model.c:109-117): dt = period /
+    ffd / substeps, dr = sqrt(0.2 dt k / (rho Cp)), ceil(depth / dr) points (at
+    least 3): 467 at the reference's 72 longitudes, 1477 at 720."""
+    dt = 9.07417 * 3600.0 / ffd / tp.n_substep
+    dr = math.sqrt(0.1 * 2.0 * dt * tp.conductivity / tp.density / tp.Cp)
+    return max(3, math.ceil(tp.depth / dr))
+
+
 def thermal_run(lib) -> dict:
-    """The thermal model (thermal_model.c thermal_solve, default parameters: 467
-    depth points, 5 substeps) on the device at the reference's 36x72 grid and at
-    360x720; wall time of dsmc_thermal_solve (upload, kernel, result back)."""
+    """The thermal model (thermal_model.c thermal_solve, default parameters, 5
+    substeps) on the device at the reference's 36x72 grid and at 360x720; wall
+    time of dsmc_thermal_solve (scratch allocation, upload, kernel, result back).
+    Algorithmic HBM bytes per cell: the initial profile written (8 h), then per
+    substep the forward sweep reads the old profile and writes the eliminated
+    right-hand side, the back substitution reads it and writes the new profile
+    (32 h); the rate below is that traffic over the whole call's wall time, so a
+    lower bound on the kernel's own bandwidth."""
     tp = capi.ThermalParams()
     lib.thermal_params_default(C.byref(tp))
-    out = {"depth_points": 467, "substeps": int(tp.n_substep)}
+    out = {"substeps": int(tp.n_substep)}
     for ftd, ffd in THERMAL_GRIDS:
         T = np.zeros(ftd * ffd)
         ms = []
@@ -95,8 +110,13 @@ def thermal_run(lib) -> dict:
             if lib.dsmc_thermal_solve(ftd, ffd, C.byref(tp), None, T.ctypes.data_as(C.POINTER(C.c_double))):
                 raise RuntimeError("dsmc_thermal_solve failed")
             ms.append((time.perf_counter() - t0) * 1e3)
+        h = thermal_depth_points(tp, ffd)
+        traffic = (8 * h + tp.n_substep * 32 * h) * ftd * ffd
         out[f"{ftd}x{ffd}_ms"] = round(min(ms), 3)
+        out[f"{ftd}x{ffd}_depth_points"] = h
         out[f"{ftd}x{ffd}_cells_per_s"] = ftd * ffd / (min(ms) / 1e3)
+        out[f"{ftd}x{ffd}_hbm_bytes"] = traffic
+        out[f"{ftd}x{ffd}_GB_per_s"] = round(traffic / (min(ms) / 1e3) / 1e9, 1)
     return out
 
 
