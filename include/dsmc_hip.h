@@ -209,10 +209,14 @@ int dsmc_run_advanced(const dsmc_params *p, const dsmc_surface *s,
 typedef struct dsmc_launch_shape {
     int block;                /* threads per block of the flight kernel (64, 128, 192 or 256) */
     int blocks_per_cu;        /* persistent flight blocks per CU */
-    int relaunch_batch;       /* idle lanes a wavefront gathers before pulling new particles */
+    int relaunch_batch;       /* split kernels: idle lanes a wavefront gathers before pulling
+                                 new particles; service inline: lanes whose hop ended that a
+                                 wavefront gathers before running the landing service */
     int step_budget;          /* steps a particle takes per flight launch before yielding */
-    int service_inline;       /* physical runs: 1 = landings serviced inside the flight
-                                 kernel, 0 = by a separate service kernel per round */
+    int service_inline;       /* 1 = landings serviced inside the flight kernel (physical and
+                                 compat ballistic runs), 0 = by a separate service kernel per
+                                 round (physical advanced runs are always serviced inline,
+                                 compat advanced runs always by the separate kernel) */
 } dsmc_launch_shape;
 
 /* Staged runs: inputs uploaded once and kept resident in device memory, the run
@@ -221,7 +225,9 @@ typedef struct dsmc_launch_shape {
 typedef struct dsmc_run dsmc_run;
 dsmc_run *dsmc_run_create(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *outputs,
                           int *status);
-int dsmc_run_launch(dsmc_run *run, const dsmc_launch_shape *shape);  /* async on the run's stream */
+/* Async on the run's stream; shape NULL = the default for the run's mode, body
+ * and size. */
+int dsmc_run_launch(dsmc_run *run, const dsmc_launch_shape *shape);
 int dsmc_run_sync(dsmc_run *run);
 double dsmc_run_kernel_ms(dsmc_run *run);        /* HIP-event time of the last launch */
 /* Device time spent in the flight kernel and number of rounds of the last launch. */
