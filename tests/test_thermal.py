@@ -11,6 +11,8 @@
 from __future__ import annotations
 
 import ctypes as C
+import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -141,3 +143,22 @@ def test_gpu_thermal_bitwise(orc, dev):
     g = SphGrid(36, 72, 1, None, temp.ctypes.data_as(DP), None, None)
     lib.thermal_solve(C.byref(g), C.byref(tp), None, 4)
     assert np.array_equal(temp.view(np.uint64), run_oracle_thermal(dev, tp, None, 36, 72).view(np.uint64))
+
+
+@pytest.mark.gpu
+def test_reference_cli_thermal_on_the_library(orc, dev, tmp_path):
+    """The reference CLI (src/main.c, unmodified, linked against this library
+    alone) as `thermal --za 20` (main.c:117-144: grid_create, thermal_solve,
+    the T file written by the CLI itself): the file holds the device-math
+    oracle's temperatures ("%8.6f"), and --za does not change them (z_angle is
+    not read by the solve)."""
+    cli = ROOT / "oracle" / "_ref" / "dsmc_cli"
+    if not cli.exists():
+        pytest.skip("the reference CLI (make -C oracle ref) is not built")
+    print("reference CLI: thermal", flush=True)
+    r = subprocess.run([str(cli), "thermal", "--za", "20", "-t", "4", "-o", str(tmp_path / "r")],
+                       capture_output=True, text=True, timeout=300, env=dict(os.environ))
+    assert r.returncode == 0, r.stderr
+    assert "Thermal model done." in r.stdout
+    want = run_oracle_thermal(dev, defaults(orc), None, 36, 72)
+    assert (tmp_path / "r_temperature.dat").read_text() == "36\n72\n" + "".join("%8.6f\n" % v for v in want)
