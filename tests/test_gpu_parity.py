@@ -351,6 +351,34 @@ def test_reference_cli_pipeline_on_the_library(lib, tmp_path):
     assert nr == no and nr > 0 and cr == co
 
 
+def test_reference_cli_dsmc_commands_on_the_library(lib, tmp_path):
+    """The reference CLI's `dsmc -n -T -o -t` and `dsmc-adv -n -T -d -v -o -t`
+    (main.c:146-151) on this library against run_ballistic / run_advanced called
+    directly with the same arguments: ttime files byte for byte, densities to
+    the last printed digit, the same velocity-record count in every cell."""
+    if not REF_CLI.exists():
+        pytest.skip("the reference CLI (make -C oracle ref) is not built")
+    T = str(GOLDEN / "T_thermal.dat")
+    r, o, r2, o2 = (str(tmp_path / x) for x in ("r", "o", "r2", "o2"))
+    env = {**os.environ, "DSMC_SEED": "9"}
+    print("reference CLI: dsmc, dsmc-adv", flush=True)
+    for args in (["dsmc", "-n", "250", "-T", T, "-t", "4", "-o", r],
+                 ["dsmc-adv", "-n", "250", "-T", T, "-d", f"{r}_den.dat", "-v", f"{r}_v_record.dat", "-t", "4",
+                  "-o", r2]):
+        cli = subprocess.run([str(REF_CLI), *args], capture_output=True, text=True, timeout=300, env=env)
+        assert cli.returncode == 0, cli.stderr
+    run_python(f"lib.run_ballistic(250, {T!r}.encode(), {o!r}.encode(), 4)", seed=9)
+    run_python(f"lib.run_advanced(250, {T!r}.encode(), {(r + '_den.dat')!r}.encode(), "
+               f"{(r + '_v_record.dat')!r}.encode(), {o2!r}.encode(), 4)", seed=9)
+    for a, b in (("r_ttime", "o_ttime"), ("r2_ttime", "o2_ttime")):
+        assert (tmp_path / f"{a}.dat").read_bytes() == (tmp_path / f"{b}.dat").read_bytes(), a
+    for a, b in (("r_den", "o_den"), ("r_horizon", "o_horizon"), ("r2_den", "o2_den"), ("r2_horizon", "o2_horizon")):
+        x = np.array((tmp_path / f"{a}.dat").read_text().split(), float)
+        y = np.array((tmp_path / f"{b}.dat").read_text().split(), float)
+        assert len(x) == len(y) and np.max(np.abs(x - y)) <= 2e-6, a
+    assert vrec_cells(tmp_path / "r_v_record.dat") == vrec_cells(tmp_path / "o_v_record.dat")
+
+
 def test_fits_bitwise(lib, dev):
     """Device binning (counting sort) + per-cell reverse-CDF polyfit ==
     advanced.c:178-209 + :32-117 as restated by the oracle."""
