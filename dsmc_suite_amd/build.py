@@ -66,6 +66,10 @@ def build(verbose: bool = False, oracle: bool = True) -> Path:
     subprocess.run([cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs)],
                    check=True)
     os.replace(tmp, LIB)
+    # the tests' command-line driver (the reference CLI's calls, linked to the library alone)
+    cli = ROOT / "tests" / "cli"
+    subprocess.run(["gcc", "-O2", "-std=gnu11", "-Wall", "-Wno-format-truncation", "-o", str(cli / "dsmc_cli"), str(cli / "dsmc_cli.c"),
+                    f"-L{PKG}", "-ldsmc_hip", "-Wl,-rpath,$ORIGIN/../../dsmc_suite_amd", "-lm"], check=True)
     if oracle:
         subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
         if Path("/root/reference/src/dsmc/ballistic.c").exists():

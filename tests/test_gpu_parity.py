@@ -310,7 +310,7 @@ def test_pipeline_hands_over_what_the_files_hold(lib, tmp_path):
     assert np.all(tt[8] == 0) and np.all(tt[4] >= 0)
 
 
-REF_CLI = capi.PKG_DIR.parent / "oracle" / "_ref" / "dsmc_cli"
+CLI = capi.PKG_DIR.parent / "tests" / "cli" / "dsmc_cli"
 
 
 def vrec_cells(path):
@@ -322,19 +322,19 @@ def vrec_cells(path):
     return n, {tuple(c): k for c, k in zip(cells, counts)}
 
 
-def test_reference_cli_pipeline_on_the_library(lib, tmp_path):
-    """The reference CLI (src/main.c, unmodified) linked against this library
-    alone, run as `pipeline -n 300 -t 4` (main.c:153-204: thermal model, T file,
-    run_ballistic, run_advanced on the phase-1 files), against dsmc_pipeline:
+def test_cli_pipeline_on_the_library(lib, tmp_path):
+    """The tests' C driver (tests/cli/dsmc_cli.c: the calls of the reference
+    CLI, src/main.c:153-204, linked against this library alone) run as
+    `pipeline -n 300 -t 4` (thermal model, T file, run_ballistic, run_advanced
+    on the phase-1 files), against dsmc_pipeline:
     the T file and both phases' ttime files byte for byte, densities and
     horizons to the last printed digit (atomics add in any order), the same
     number of velocity records in every (td, ht) cell (which records fill a
     capped cell is first come, first kept)."""
-    if not REF_CLI.exists():
-        pytest.skip("the reference CLI (make -C oracle ref) is not built")
+    assert CLI.exists(), "tests/cli/dsmc_cli is not built (__graft_entry__.build())"
     r, o = str(tmp_path / "r"), str(tmp_path / "o")
-    print("reference CLI: pipeline, 300 particles", flush=True)
-    cli = subprocess.run([str(REF_CLI), "pipeline", "-n", "300", "-t", "4", "-o", r], capture_output=True,
+    print("CLI driver: pipeline, 300 particles", flush=True)
+    cli = subprocess.run([str(CLI), "pipeline", "-n", "300", "-t", "4", "-o", r], capture_output=True,
                          text=True, timeout=300, env={**os.environ, "DSMC_SEED": "6"})
     assert cli.returncode == 0, cli.stderr
     assert "=== PIPELINE COMPLETE ===" in cli.stdout
@@ -351,21 +351,20 @@ def test_reference_cli_pipeline_on_the_library(lib, tmp_path):
     assert nr == no and nr > 0 and cr == co
 
 
-def test_reference_cli_dsmc_commands_on_the_library(lib, tmp_path):
-    """The reference CLI's `dsmc -n -T -o -t` and `dsmc-adv -n -T -d -v -o -t`
-    (main.c:146-151) on this library against run_ballistic / run_advanced called
+def test_cli_dsmc_commands_on_the_library(lib, tmp_path):
+    """The C driver's `dsmc -n -T -o -t` and `dsmc-adv -n -T -d -v -o -t` (the
+    reference CLI's calls, main.c:146-151) against run_ballistic / run_advanced called
     directly with the same arguments: ttime files byte for byte, densities to
     the last printed digit, the same velocity-record count in every cell."""
-    if not REF_CLI.exists():
-        pytest.skip("the reference CLI (make -C oracle ref) is not built")
+    assert CLI.exists(), "tests/cli/dsmc_cli is not built (__graft_entry__.build())"
     T = str(GOLDEN / "T_thermal.dat")
     r, o, r2, o2 = (str(tmp_path / x) for x in ("r", "o", "r2", "o2"))
     env = {**os.environ, "DSMC_SEED": "9"}
-    print("reference CLI: dsmc, dsmc-adv", flush=True)
+    print("CLI driver: dsmc, dsmc-adv", flush=True)
     for args in (["dsmc", "-n", "250", "-T", T, "-t", "4", "-o", r],
                  ["dsmc-adv", "-n", "250", "-T", T, "-d", f"{r}_den.dat", "-v", f"{r}_v_record.dat", "-t", "4",
                   "-o", r2]):
-        cli = subprocess.run([str(REF_CLI), *args], capture_output=True, text=True, timeout=300, env=env)
+        cli = subprocess.run([str(CLI), *args], capture_output=True, text=True, timeout=300, env=env)
         assert cli.returncode == 0, cli.stderr
     run_python(f"lib.run_ballistic(250, {T!r}.encode(), {o!r}.encode(), 4)", seed=9)
     run_python(f"lib.run_advanced(250, {T!r}.encode(), {(r + '_den.dat')!r}.encode(), "

@@ -141,19 +141,19 @@ def test_gpu_shadow_flux_bitwise(tmp_path):
 
 
 @pytest.mark.gpu
-def test_reference_cli_raytrace_on_the_library(tmp_path):
-    """The reference CLI (src/main.c, unmodified, linked against this library
-    alone) as `raytrace -m v -f f --npoint --nface --za --tt --nmonte`
-    (main.c:107-116): its flux file equals the oracle's device-math build."""
-    cli = capi.PKG_DIR.parent / "oracle" / "_ref" / "dsmc_cli"
-    if not cli.exists():
-        pytest.skip("the reference CLI (make -C oracle ref) is not built")
+def test_cli_raytrace_on_the_library(tmp_path):
+    """The tests' C driver (tests/cli/dsmc_cli.c: the reference CLI's calls,
+    linked against this library alone) as `raytrace -m v -f f --npoint --nface
+    --za --tt --nmonte` (main.c:104-116): its flux file equals the oracle's
+    device-math build."""
+    cli = capi.PKG_DIR.parent / "tests" / "cli" / "dsmc_cli"
+    assert cli.exists(), "tests/cli/dsmc_cli is not built (__graft_entry__.build())"
     dev = oracle(devmath=True)
     vb, fb = sphere(3, noise=0.15, seed=4)
     vf, ff = tmp_path / "v.txt", tmp_path / "f.txt"
     vf.write_text("".join("%.17g %.17g %.17g\n" % tuple(x) for x in vb))
     ff.write_text("".join("%d %d %d\n" % tuple(x + 1) for x in fb))
-    print("reference CLI: raytrace", flush=True)
+    print("CLI driver: raytrace", flush=True)
     r = subprocess.run([str(cli), "raytrace", "-m", str(vf), "-f", str(ff), "--npoint", str(len(vb)),
                         "--nface", str(len(fb)), "--za", "30", "--tt", "5", "--nmonte", "24", "-t", "4",
                         "-o", str(tmp_path / "r")], capture_output=True, text=True, timeout=300, env=dict(os.environ))

@@ -146,16 +146,15 @@ def test_gpu_thermal_bitwise(orc, dev):
 
 
 @pytest.mark.gpu
-def test_reference_cli_thermal_on_the_library(orc, dev, tmp_path):
-    """The reference CLI (src/main.c, unmodified, linked against this library
-    alone) as `thermal --za 20` (main.c:117-144: grid_create, thermal_solve,
-    the T file written by the CLI itself): the file holds the device-math
-    oracle's temperatures ("%8.6f"), and --za does not change them (z_angle is
-    not read by the solve)."""
-    cli = ROOT / "oracle" / "_ref" / "dsmc_cli"
-    if not cli.exists():
-        pytest.skip("the reference CLI (make -C oracle ref) is not built")
-    print("reference CLI: thermal", flush=True)
+def test_cli_thermal_on_the_library(orc, dev, tmp_path):
+    """The tests' C driver (tests/cli/dsmc_cli.c: the reference CLI's calls,
+    linked against this library alone) as `thermal --za 20` (main.c:117-144:
+    grid_create, thermal_solve, the T file written by the driver): the file
+    holds the device-math oracle's temperatures ("%8.6f"), and --za does not
+    change them (z_angle is not read by the solve)."""
+    cli = ROOT / "tests" / "cli" / "dsmc_cli"
+    assert cli.exists(), "tests/cli/dsmc_cli is not built (__graft_entry__.build())"
+    print("CLI driver: thermal", flush=True)
     r = subprocess.run([str(cli), "thermal", "--za", "20", "-t", "4", "-o", str(tmp_path / "r")],
                        capture_output=True, text=True, timeout=300, env=dict(os.environ))
     assert r.returncode == 0, r.stderr
