@@ -24,7 +24,7 @@ ARCH = os.environ.get("DSMC_ARCH", "gfx950")
 
 HIP_SOURCES = ["kernels.hip", "host.hip", "advanced.hip", "pool.hip", "batch.hip", "serve.hip", "thermal.hip",
                "shadow.hip"]
-CXX_SOURCES = ["io.cpp", "surface.cpp"]
+CXX_SOURCES = ["io.cpp", "surface.cpp", "common_api.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-munsafe-fp-atomics",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

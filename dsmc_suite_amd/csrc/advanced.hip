@@ -20,6 +20,7 @@
 
 #include "../../include/dsmc_hip.h"
 #include "dsmc_lane.h"
+#include "dsmc_polyfit.h"
 #include "dsmc_kernels.h"
 #include "dsmc_host.h"
 
@@ -83,51 +84,6 @@ __global__ void scatter_kernel(const int32_t *cell, int64_t n, const int32_t *st
     bucket[pos] = (int32_t)i;
 }
 
-/* polyfit.c:10-101, written for one lane. */
-__device__ int polyfit_dev(const double *x, const double *y, int n, int degree, double *coeffs)
-{
-    int m = degree + 1;
-    if (n < m) return -1;
-    double S[14];
-    for (int k = 0; k < 14; k++) S[k] = 0.0;
-    for (int i = 0; i < n; i++) {
-        double xp = 1.0;
-        for (int k = 0; k <= 2 * degree; k++) { S[k] += xp; xp *= x[i]; }
-    }
-    double a[7], rhs[7];
-    for (int j = 0; j < 7; j++) { a[j] = 0.0; rhs[j] = 0.0; }
-    for (int i = 0; i < n; i++) {
-        double xp = 1.0;
-        for (int j = 0; j < m; j++) { rhs[j] += xp * y[i]; xp *= x[i]; }
-    }
-    double mat[7][8];
-    for (int r = 0; r < 7; r++) for (int c = 0; c < 8; c++) mat[r][c] = 0.0;
-    for (int r = 0; r < m; r++) {
-        for (int c = 0; c < m; c++) mat[r][c] = S[r + c];
-        mat[r][m] = rhs[r];
-    }
-    for (int col = 0; col < m; col++) {
-        int max_row = col;
-        double max_val = fabs(mat[col][col]);
-        for (int row = col + 1; row < m; row++)
-            if (fabs(mat[row][col]) > max_val) { max_val = fabs(mat[row][col]); max_row = row; }
-        if (max_val < 1e-30) return -1;
-        if (max_row != col)
-            for (int k = 0; k <= m; k++) { double t = mat[col][k]; mat[col][k] = mat[max_row][k]; mat[max_row][k] = t; }
-        for (int row = col + 1; row < m; row++) {
-            double factor = mat[row][col] / mat[col][col];
-            for (int k = col; k <= m; k++) mat[row][k] -= factor * mat[col][k];
-        }
-    }
-    for (int row = m - 1; row >= 0; row--) {
-        a[row] = mat[row][m];
-        for (int col = row + 1; col < m; col++) a[row] -= mat[row][col] * a[col];
-        a[row] /= mat[row][row];
-    }
-    for (int i = 0; i < m; i++) coeffs[i] = a[degree - i];
-    return 0;
-}
-
 __global__ void __launch_bounds__(FIT_BLOCK)
 fit_kernel(const int32_t *hist, const int32_t *start, const int32_t *bucket, const double *vrec, int ncell,
            int max_per_cell, int degree, int min_samples, int32_t *count_num, double *coeffs,
@@ -174,7 +130,7 @@ fit_kernel(const int32_t *hist, const int32_t *start, const int32_t *bucket, con
         for (int j = i; j < nbins; j++) yy2[i] += yy[j];
     }
     double cf[7];
-    if (polyfit_dev(xx, yy2, nbins, degree, cf) == 0) {
+    if (lsq_poly(xx, yy2, nbins, degree, cf) == 0) {
         for (int i = 0; i <= degree; i++) coeffs[(size_t)c * (degree + 1) + i] = cf[i];
         pmin[c] = min_v;
         pmax[c] = max_v;
@@ -210,7 +166,7 @@ __global__ void hist_fit_kernel(const unsigned long long *vhist, int ncell, int 
         x[k - lo] = ((double)(k - lo) + 0.5) / (double)npts;
         y[k - lo] = (double)DSMC_VIRTUAL_RECORDS * (double)acc / (double)N;
     }
-    if (polyfit_dev(x, y, npts, degree, cf) != 0) return;
+    if (lsq_poly(x, y, npts, degree, cf) != 0) return;
     for (int i = 0; i <= degree; i++) coeffs[(size_t)c * (degree + 1) + i] = cf[i];
     pmin[c] = (double)lo * dv;
     pmax[c] = (double)(hi + 1) * dv;

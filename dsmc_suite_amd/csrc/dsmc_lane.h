@@ -599,6 +599,33 @@ DSMC_HD void lane_begin_flight(DsmcLane &L, const DsmcEnv &E, bool near = false)
     L.phase = LANE_FLIGHT;
 }
 
+/* ------------------------------------------------------------------ velocity table use */
+/* velocity.c:49-68: the table row of a local temperature (rows 21-22 are never
+ * chosen: every T >= 220 K takes row 20). */
+DSMC_HD int vt_row(double T)
+{
+    int idx = T >= 220.0 ? 20 : (int)ceil(T / 10.0) - 2;
+    if (idx < 0) idx = 0;
+    if (idx >= DSMC_VT_NT) idx = DSMC_VT_NT - 1;
+    return idx;
+}
+
+/* velocity.c:70-73: outgoing speed with sticking coefficient alpha. */
+DSMC_HD double sticking_speed(double vr, double v_old_sq, double alpha)
+{
+    return sqrt(alpha * vr * vr + (1.0 - alpha) * v_old_sq);
+}
+
+/* collision.c:15-41 as the reference evaluates it: v <- (v + v2) + 0.5 (v - v2). */
+DSMC_HD void collide_compat(double &vx, double &vy, double &vz, double v2x, double v2y, double v2z)
+{
+    const double dvx = vx - v2x, dvy = vy - v2y, dvz = vz - v2z;
+    const double cvx = vx + v2x, cvy = vy + v2y, cvz = vz + v2z;
+    vx = cvx + 0.5 * dvx;
+    vy = cvy + 0.5 * dvy;
+    vz = cvz + 0.5 * dvz;
+}
+
 /* ------------------------------------------------------------------ compat launch */
 /* Body of `while (W >= 0.01)` up to the flight loop: ballistic.c:124-186 and
  * advanced.c:271-338. */
@@ -652,16 +679,11 @@ DSMC_HD void lane_launch_compat(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
         }
         local_T = E.T[td * ffd + fd];
     }
-    /* velocity.c:49-68 */
-    int idx;
-    if (local_T >= 220.0) idx = 20;
-    else idx = (int)ceil(local_T / 10.0) - 2;
-    if (idx < 0) idx = 0;
-    if (idx >= DSMC_VT_NT) idx = DSMC_VT_NT - 1;
+    const int idx = vt_row(local_T);                                      /* velocity.c:49-68 */
     int r = lane_draw(L, E) % E.vt_count[idx];
     double vr = E.vt_speeds[idx * DSMC_VT_NV + r];
     double v_old_sq = L.vx * L.vx + L.vy * L.vy + L.vz * L.vz;
-    double vout = sqrt(E.alpha * vr * vr + (1.0 - E.alpha) * v_old_sq);   /* velocity.c:70-73 */
+    double vout = sticking_speed(vr, v_old_sq, E.alpha);
     double flyd = lane_urand(L, E) * M_PI + L.thi - M_PI / 2.0;
     double xi = lane_urand(L, E) * M_PI + L.fi - M_PI / 2.0;
     double sfl, cfl, sxi, cxi, sr, cr;
@@ -1043,6 +1065,31 @@ DSMC_HD double sample_speed(const double *cg, int deg, double min_v, double max_
     return (v0 + v1) * 0.5;
 }
 
+/* integrator.c:4-55: one Stormer-Verlet step of position s and velocity v under
+ * the attraction GM / r^2, with the previous step's acceleration o (replaced by
+ * this step's); Ri = |s|, c1 = 1.5 h^2, c2 = h^2 / 6.  DS: *ds = |s_new - s|.
+ * Returns |s_new|.  The reference's operation order, no contraction. */
+template <bool DS>
+DSMC_HD double verlet_push(double &sx, double &sy, double &sz, double &vx, double &vy, double &vz, double &ox,
+                           double &oy, double &oz, double Ri, double GM, double h, double c1, double c2, double *ds)
+{
+    const double g = GM / (Ri * Ri * Ri);
+    const double ax = g * sx, ay = g * sy, az = g * sz;
+    const double nx = sx + vx * h - ax * c1 - ox * c2;
+    const double ny = sy + vy * h - ay * c1 - oy * c2;
+    const double nz = sz + vz * h - az * c1 - oz * c2;
+    if (DS) {
+        const double dx = nx - sx, dy = ny - sy, dz = nz - sz;
+        *ds = sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    vx -= ax * h;
+    vy -= ay * h;
+    vz -= az * h;
+    ox = ax; oy = ay; oz = az;
+    sx = nx; sy = ny; sz = nz;
+    return sqrt(nx * nx + ny * ny + nz * nz);
+}
+
 template <class Ops, int ADV>
 DSMC_HD void lane_step_compat(DsmcLane &L, const DsmcEnv &E, const DsmcCellTabs &TB, double nx, double ny,
                               double nz, double Rn, double ds);
@@ -1058,24 +1105,10 @@ DSMC_HD void lane_step(DsmcLane &L, const EnvT &E, const DsmcCellTabs &TB)
     if constexpr (MODE != 0) CP = cell_pre(L.td, L.fd, L.ht, TB.tq, TB.cs, TB.hr);
     if (MODE == 0) { L.flight_time += h; L.time += h; }
     L.steps++;
-    double Ri = L.Ri;
-    double Ri3 = Ri * Ri * Ri;
-    double g = E.GM / Ri3;
-    double ax = g * L.sx, ay = g * L.sy, az = g * L.sz;
-    double nx = L.sx + L.vx * h - ax * E.c1 - L.ox * E.c2;
-    double ny = L.sy + L.vy * h - ay * E.c1 - L.oy * E.c2;
-    double nz = L.sz + L.vz * h - az * E.c1 - L.oz * E.c2;
     double ds = 0.0;
-    if (ADV) {
-        double dx = nx - L.sx, dy = ny - L.sy, dz = nz - L.sz;
-        ds = sqrt(dx * dx + dy * dy + dz * dz);
-    }
-    L.vx -= ax * h;
-    L.vy -= ay * h;
-    L.vz -= az * h;
-    L.ox = ax; L.oy = ay; L.oz = az;
-    L.sx = nx; L.sy = ny; L.sz = nz;
-    double Rn = sqrt(nx * nx + ny * ny + nz * nz);
+    const double Rn = verlet_push<ADV != 0>(L.sx, L.sy, L.sz, L.vx, L.vy, L.vz, L.ox, L.oy, L.oz, L.Ri, E.GM, h,
+                                            E.c1, E.c2, &ds);
+    const double nx = L.sx, ny = L.sy, nz = L.sz;
     L.Ri = Rn;
     if (Rn <= E.R) { L.phase = LANE_LANDED; return; }
     if (Rn >= E.max_height) { lane_flush<Ops>(L, E); L.fate = 1; L.phase = LANE_ENDED; return; }
@@ -1140,12 +1173,7 @@ DSMC_HD void lane_step_compat(DsmcLane &L, const DsmcEnv &E, const DsmcCellTabs 
                     double st2, ct2, sf2, cf2;
                     dm_sincos(thi2, &st2, &ct2);
                     dm_sincos(fi2, &sf2, &cf2);
-                    double v2x = vv2 * ct2 * cf2, v2y = vv2 * ct2 * sf2, v2z = vv2 * st2;
-                    double dvx = L.vx - v2x, dvy = L.vy - v2y, dvz = L.vz - v2z;   /* collision.c:35-40 */
-                    double cvx = L.vx + v2x, cvy = L.vy + v2y, cvz = L.vz + v2z;
-                    L.vx = cvx + 0.5 * dvx;
-                    L.vy = cvy + 0.5 * dvy;
-                    L.vz = cvz + 0.5 * dvz;
+                    collide_compat(L.vx, L.vy, L.vz, vv2 * ct2 * cf2, vv2 * ct2 * sf2, vv2 * st2);
                     return;
                 }
             }
