@@ -811,17 +811,13 @@ static void phys_collide(const ctx_t *c, V3 *v, size_t cell, orng *rg)
     v->x = out[0]; v->y = out[1]; v->z = out[2];
 }
 
-static void physical_particle(ctx_t *c, int64_t n, orng *rg)
+/* A particle's launch: face by inverse CDF of the sublimation weights, a point
+ * uniform on the flat triangle (square-root barycentric map) projected to radius
+ * R, its photolysis step count ceil(E tau / dt) with E ~ Exp(1), then the
+ * flux-Maxwellian emission at the face temperature. */
+static void phys_launch(const ctx_t *c, orng *rg, int *face_out, V3 *s_out, V3 *v_out, int64_t *n_photo)
 {
-    const dsmc_params *P = c->p;
-    const int ftd = c->ftd, ffd = c->ffd, hd = c->hd;
-    const double h = c->h, R = c->R;
     const int nf = c->nfine;
-    int fate = DSMC_FATE_NONE, hops = 0;
-    int64_t steps = 0, n_day = 0;
-    const double W = 1.0;
-
-    /* launch face by inverse CDF of the sublimation weights */
     double x = orng_u01(rg) * c->cdf[nf - 1];
     int lo = 0, hi = nf - 1;
     while (lo < hi) { int mid = (lo + hi) / 2; if (c->cdf[mid] > x) hi = mid; else lo = mid + 1; }
@@ -833,12 +829,29 @@ static void physical_particle(ctx_t *c, int64_t n, orng *rg)
     double wa = 1.0 - r1, wb = r1 * (1.0 - u2), wc = r1 * u2;
     V3 pt = {wa * a.x + wb * b.x + wc * e.x, wa * a.y + wb * b.y + wc * e.y,
              wa * a.z + wb * b.z + wc * e.z};
-    double sc = R / v3len(pt);
+    double sc = c->R / v3len(pt);
     V3 s = {pt.x * sc, pt.y * sc, pt.z * sc};
     double E = -OM_LOG(orng_u01(rg));
-    double nph = ceil(E * P->lifetime / h);
-    int64_t n_photo = nph > 9.0e18 ? (int64_t)9000000000000000000LL : (int64_t)nph;
-    V3 v = emit(c, s, c->surf->face_T[face], rg);
+    double nph = ceil(E * c->p->lifetime / c->h);
+    *n_photo = nph > 9.0e18 ? (int64_t)9000000000000000000LL : (int64_t)nph;
+    *face_out = face;
+    *s_out = s;
+    *v_out = emit(c, s, c->surf->face_T[face], rg);
+}
+
+static void physical_particle(ctx_t *c, int64_t n, orng *rg)
+{
+    const dsmc_params *P = c->p;
+    const int ftd = c->ftd, ffd = c->ffd, hd = c->hd;
+    const double h = c->h, R = c->R;
+    int fate = DSMC_FATE_NONE, hops = 0;
+    int64_t steps = 0, n_day = 0;
+    const double W = 1.0;
+
+    int face;
+    V3 s, v;
+    int64_t n_photo;
+    phys_launch(c, rg, &face, &s, &v, &n_photo);
     double tau = 0.0, tau_next = 0.0;          /* collision depth and its threshold */
     int64_t ncol = 0;
     if (c->coll) tau_next = -OM_LOG(orng_u01(rg));
@@ -1001,6 +1014,76 @@ static void zero_result(const dsmc_params *p, dsmc_result *r, int ttime_rows)
     r->vrec_n = 0;
 }
 
+/* The icosphere part of a physical run's context: faces level by level, the
+ * launch CDF, the sun direction and the rotation rate.  c->cdf is malloc'ed. */
+static void phys_surface(ctx_t *c, const dsmc_params *p, const dsmc_surface *s)
+{
+    c->surf = s;
+    int off = 0, nl = 20;
+    for (int l = 0; l <= s->levels; l++) {
+        c->lvl_faces[l] = s->faces + 3 * (size_t)off;
+        c->lvl_n[l] = nl;
+        off += nl;
+        nl *= 4;
+    }
+    c->nfine = c->lvl_n[s->levels];
+    c->cdf = malloc(sizeof(double) * c->nfine);
+    double acc = 0.0;
+    for (int i = 0; i < c->nfine; i++) { acc += s->face_w[i]; c->cdf[i] = acc; }
+    c->sun_x = OM_SIN(p->subsolar_lon);
+    c->sun_y = OM_COS(p->subsolar_lon);
+    c->omega = 2.0 * M_PI / p->period;
+}
+
+/* Test hooks of the physical model's parts, for their independent checks:
+ * the launches of particles 0..n-1 (their keyed streams, as in a run): face,
+ * position [n*3], velocity [n*3], photolysis step count; and the face the
+ * hierarchical search finds for directions d [n*3]. */
+int cpu_ref_phys_launch(const dsmc_params *p, const dsmc_surface *s, int64_t n, int32_t *face, double *pos,
+                        double *vel, int64_t *n_photo)
+{
+    if (!p || !s || n < 0 || !s->vert || !s->faces || !s->face_T || !s->face_w || s->levels < 0 || s->levels > 10)
+        return DSMC_ERR_ARG;
+    ctx_t c;
+    dsmc_result none;
+    memset(&none, 0, sizeof none);
+    ctx_common(&c, p, &none);
+    phys_surface(&c, p, s);
+    orng g;
+    orng_init(&g, ORNG_KEYED, p, 0, 0u, 0u);
+    for (int64_t i = 0; i < n; i++) {
+        g.pid = (uint64_t)i;
+        g.k = 0;
+        int f;
+        V3 x, v;
+        int64_t k;
+        phys_launch(&c, &g, &f, &x, &v, &k);
+        face[i] = f;
+        pos[3 * i] = x.x; pos[3 * i + 1] = x.y; pos[3 * i + 2] = x.z;
+        vel[3 * i] = v.x; vel[3 * i + 1] = v.y; vel[3 * i + 2] = v.z;
+        n_photo[i] = k;
+    }
+    free(c.cdf);
+    return DSMC_OK;
+}
+
+int cpu_ref_locate_face(const dsmc_params *p, const dsmc_surface *s, const double *d, int64_t n, int32_t *out)
+{
+    if (!p || !s || n < 0 || !s->vert || !s->faces || !s->face_w || s->levels < 0 || s->levels > 10)
+        return DSMC_ERR_ARG;
+    ctx_t c;
+    dsmc_result none;
+    memset(&none, 0, sizeof none);
+    ctx_common(&c, p, &none);
+    phys_surface(&c, p, s);
+    for (int64_t i = 0; i < n; i++) {
+        V3 x = {d[3 * i], d[3 * i + 1], d[3 * i + 2]};
+        out[i] = locate_face(&c, x);
+    }
+    free(c.cdf);
+    return DSMC_OK;
+}
+
 int cpu_ref_ballistic(const dsmc_params *p, const dsmc_surface *s, dsmc_result *r,
                       int nthreads, int rng_mode)
 {
@@ -1036,21 +1119,7 @@ int cpu_ref_ballistic(const dsmc_params *p, const dsmc_surface *s, dsmc_result *
     } else {
         if (!s->vert || !s->faces || !s->face_T || !s->face_w || s->levels < 0 || s->levels > 10)
             return DSMC_ERR_ARG;
-        c.surf = s;
-        int off = 0, nl = 20;
-        for (int l = 0; l <= s->levels; l++) {
-            c.lvl_faces[l] = s->faces + 3 * (size_t)off;
-            c.lvl_n[l] = nl;
-            off += nl;
-            nl *= 4;
-        }
-        c.nfine = c.lvl_n[s->levels];
-        c.cdf = malloc(sizeof(double) * c.nfine);
-        double acc = 0.0;
-        for (int i = 0; i < c.nfine; i++) { acc += s->face_w[i]; c.cdf[i] = acc; }
-        c.sun_x = OM_SIN(p->subsolar_lon);
-        c.sun_y = OM_COS(p->subsolar_lon);
-        c.omega = 2.0 * M_PI / p->period;
+        phys_surface(&c, p, s);
         if (r->vhist) {
             if (p->vhist_bins <= 0 || p->vhist_bins > 64 || !(p->vhist_max > 0)) return DSMC_ERR_ARG;
             memset(r->vhist, 0, cells * p->vhist_bins * sizeof(int64_t));
