@@ -265,7 +265,7 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
      * 1e7: 203.6 against 210.5 ms per run with 4096 throughout, 258-282 ms with
      * 8192-20000 in the later rounds) */
     const int64_t sh_budget = sh.step_budget;
-    const int64_t late_budget = !shape && !inline_svc && sh_budget > 1024 ? 1024 : sh_budget;
+    const int64_t late_budget = sh.late_step_budget > 0 ? sh.late_step_budget : sh_budget;
     /* per-launch timing events, destroyed on every return path */
     struct Events {
         std::vector<hipEvent_t> v;
@@ -318,6 +318,7 @@ dsmc_launch_shape default_shape(const DsmcEnv &E, int short_hops)
 {
     dsmc_launch_shape s;
     s.block = 256;
+    s.late_step_budget = 0;
     if (E.mode == 0 && !E.advanced) {
         /* compat ballistic particles make hundreds of hops, many of one step:
          * with the service in its own kernel every hop is a round of launches
@@ -360,6 +361,7 @@ dsmc_launch_shape default_shape(const DsmcEnv &E, int short_hops)
         s.relaunch_batch = 16;
         s.step_budget = 4096;
         s.service_inline = 0;
+        s.late_step_budget = 1024;
     }
     return s;
 }
@@ -368,7 +370,8 @@ bool shape_ok(const dsmc_launch_shape &s)
 {
     return s.block >= 64 && s.block <= 256 && s.block % 64 == 0 && s.blocks_per_cu > 0 &&
            s.blocks_per_cu * s.block <= 2048 && s.relaunch_batch >= 1 && s.relaunch_batch <= 64 &&
-           s.step_budget >= 16 && (s.service_inline == 0 || s.service_inline == 1);
+           s.step_budget >= 16 && (s.service_inline == 0 || s.service_inline == 1) &&
+           (s.late_step_budget == 0 || s.late_step_budget >= 16);
 }
 
 int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st)

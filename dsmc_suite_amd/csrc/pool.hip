@@ -276,6 +276,7 @@ int pool_create(Pool *P, const PoolClass &c, int max_runs, int64_t max_slots, in
     sh.relaunch_batch = 48;     /* lanes serviced together with the landing service inline */
     sh.step_budget = 1024;
     sh.service_inline = 1;
+    sh.late_step_budget = 0;
     return pool_set_shape(P, shape ? *shape : sh);
 }
 

@@ -217,6 +217,8 @@ typedef struct dsmc_launch_shape {
                                  compat ballistic runs), 0 = by a separate service kernel per
                                  round (physical advanced runs are always serviced inline,
                                  compat advanced runs always by the separate kernel) */
+    int late_step_budget;     /* single runs with the separate service kernel: the step budget of
+                                 every round after the first two; 0 = step_budget (pools ignore it) */
 } dsmc_launch_shape;
 
 /* Staged runs: inputs uploaded once and kept resident in device memory, the run

@@ -78,21 +78,91 @@ def test_batch_runs_equal_single_runs(lib):
         assert_same_run(b1, c1, bufs[i], cb, f"run {i} ({sp[i].body.name}, n={sp[i].n})")
 
 
-def test_batch_run_matches_oracle(lib):
-    """One run of a batch against the CPU oracle (device-math build)."""
+def oracle_ballistic(orc, p, surf):
+    ob = Buffers(p, 5, per_particle=True, moments=True)
+    assert orc.cpu_ref_ballistic(C.byref(p), C.byref(surf.surface), C.byref(ob.result), 8, 0) == 0
+    return ob, {k: int(getattr(ob.result, k)) for k in COUNTERS}
+
+
+def oracle_collisional(orc, p1, p2, surf):
+    orc.cpu_ref_collisional.argtypes = [C.c_void_p] * 5 + [C.c_int, C.c_int]
+    w1 = Buffers(p1, 5, per_particle=False, moments=True, vhist=True)
+    w2 = Buffers(p2, 5, per_particle=True, moments=True)
+    assert orc.cpu_ref_collisional(C.byref(p1), C.byref(p2), C.byref(surf.surface), C.byref(w1.result),
+                                   C.byref(w2.result), 8, 0) == 0
+    return w2, {k: int(getattr(w2.result, k)) for k in COUNTERS}
+
+
+def assert_matches_oracle(got: Buffers, cg: dict, want: Buffers, cw: dict, what: str):
+    """HIP run against the oracle's device-math build: per-particle results,
+    deposit counts and fate counters bit for bit, velocity moments to the
+    summation-order tolerance."""
+    assert cg == cw, what
+    for f in ("end_state", "steps", "fate", "hops", "vcount"):
+        assert np.array_equal(getattr(got, f), getattr(want, f)), f"{what}: {f}"
+    cnt = np.repeat(want.vcount.astype(np.float64), 3)
+    assert np.all(np.abs(got.vmom - want.vmom) <= 1e-12 * cnt * 5e3 + 1e-9), what
+
+
+@pytest.mark.parametrize("levels", [3, 4])
+def test_batch_run_matches_oracle(lib, levels):
+    """Two runs of an autotuned batch (Ceres and the Moon) against the CPU oracle
+    (device-math build), on the 1280-face icosphere and on the batched
+    workload's 5120-face one (its 64 x 64 cube-map face table)."""
     sp = specs()[:2]
-    params, surfs = W.build_runs(lib, sp, levels=LEVELS, hd=HD)
+    params, surfs = W.build_runs(lib, sp, levels=levels, hd=HD)
     bufs = [Buffers(p, 5, per_particle=True, moments=True) for p in params]
     pa, sa = W.as_arrays(params, surfs)
     res = (capi.Result * 2)(*[b.result for b in bufs])
+    print(f"batch vs oracle, icosphere levels {levels}", flush=True)
     assert lib.dsmc_run_batch(2, pa, sa, res, None) == 0     # autotuned shape
     orc = oracle(devmath=True)
-    p = params[1]
-    ob = Buffers(p, 5, per_particle=True, moments=True)
-    assert orc.cpu_ref_ballistic(C.byref(p), C.byref(surfs[1].surface), C.byref(ob.result), 4, 0) == 0
-    for f in ("end_state", "steps", "fate", "hops", "vcount"):
-        assert np.array_equal(getattr(ob, f), getattr(bufs[1], f)), f
-    assert int(res[1].total_steps) == int(ob.result.total_steps)
+    for i in range(2):
+        ob, co = oracle_ballistic(orc, params[i], surfs[i])
+        assert_matches_oracle(bufs[i], {k: int(getattr(res[i], k)) for k in COUNTERS}, ob, co,
+                              f"run {i} ({sp[i].body.name}, {levels} levels)")
+
+
+@pytest.mark.parametrize("levels", [3, 4])
+def test_served_requests_match_oracle(lib, levels):
+    """Requests served together - two ballistic (the Moon, Mercury) and one
+    collisional (Ceres: phase 1, fits and number density on the device, phase 2)
+    - each against the CPU oracle (device-math build)."""
+    sp = [W.RunSpec(W.BODIES[1], 700, 2.0, 31), W.RunSpec(W.BODIES[3], 1200, 5.0, 32)]
+    params, surfs = W.build_runs(lib, sp, levels=levels, hd=HD)
+    csurf = W.icosphere(lib, levels, W.BODIES[0], 0.4)
+    p1, p2 = W.coll_params(lib, W.BODIES[0], 1500, 0.4, 33, hd=HD)
+    p2.nparticles = 600
+    sv = lib.dsmc_server_create(4 << 30, 8, 3)          # per-particle outputs, collisional requests
+    assert sv
+    try:
+        outs = [Buffers(p, 5, per_particle=True, moments=True) for p in params]
+        cout = Buffers(p2, 5, per_particle=True, moments=True)
+        tk = []
+        for j in range(2):
+            t = C.c_uint64()
+            assert lib.dsmc_server_submit(sv, C.byref(params[j]), C.byref(surfs[j].surface), C.byref(outs[j].result),
+                                          C.byref(t)) == 0
+            tk.append(t.value)
+            if j == 0:
+                assert lib.dsmc_server_submit_collisional(sv, C.byref(p1), C.byref(p2), C.byref(csurf.surface),
+                                                          C.byref(cout.result), C.byref(t)) == 0
+                tk.append(t.value)
+            assert lib.dsmc_server_step(sv) >= 0
+        print(f"served requests vs oracle, icosphere levels {levels}", flush=True)
+        for x in tk:
+            assert lib.dsmc_server_wait(sv, x) == 0
+    finally:
+        lib.dsmc_server_destroy(sv)
+    orc = oracle(devmath=True)
+    for j in range(2):
+        ob, co = oracle_ballistic(orc, params[j], surfs[j])
+        assert_matches_oracle(outs[j], {k: int(getattr(outs[j].result, k)) for k in COUNTERS}, ob, co,
+                              f"served ballistic request {j} ({sp[j].body.name})")
+    w2, cw = oracle_collisional(orc, p1, p2, csurf)
+    assert cw["n_collisions"] > 10, cw
+    assert_matches_oracle(cout, {k: int(getattr(cout.result, k)) for k in COUNTERS}, w2, cw,
+                          "served collisional request")
 
 
 def test_staged_batch_relaunch(lib):

@@ -214,6 +214,81 @@ def test_velocity_records_caps(lib):
     assert np.all(np.isfinite(v)) and np.all(np.linalg.norm(v, axis=1) < p.v_esc * 1.5)
 
 
+@pytest.mark.parametrize("budget,late", [(37, 23), (4096, 1024)])
+def test_default_split_path_matches_oracle(lib, dev, budget, late):
+    """The path a physical run of 4e6 particles or more takes by default
+    (flight_kernel<1,0> and the separate service kernel, 5 blocks per CU,
+    16-lane refills, deposits through the LDS slot table, velocity moments on,
+    a step budget in the first two rounds and a later one after them, open cell
+    visits carried in the records across launches), forced on 3000 Ceres
+    particles through the launch shape: per-particle results, deposit counts
+    and fate counters bit for bit against the oracle, moments to the
+    summation-order tolerance.  37/23-step launches make every visit cross many
+    launch boundaries; 4096/1024 are the default's own budgets."""
+    p = capi.preset(lib, PRESET_CERES_PHYSICAL, 36, 72, n=3000, seed=44)
+    surf = icosphere(lib, 3)
+    want = run_oracle(dev, p, surf, nthreads=8, moments=True)
+    got = Buffers(p, 5, moments=True)
+    st = C.c_int(0)
+    run = lib.dsmc_run_create(C.byref(p), C.byref(surf.surface), C.byref(got.result), C.byref(st))
+    assert run and st.value == 0
+    sh = capi.LaunchShape(256, 5, 16, budget, 0, late)
+    print(f"split path: 3000 Ceres particles, budgets {budget}/{late}", flush=True)
+    assert lib.dsmc_run_launch(run, C.byref(sh)) == 0 and lib.dsmc_run_sync(run) == 0
+    assert lib.dsmc_run_collect(run, C.byref(got.result)) == 0
+    rounds = C.c_int()
+    assert lib.dsmc_run_timing(run, None, C.byref(rounds)) == 0
+    lib.dsmc_run_destroy(run)
+    assert rounds.value >= (20 if budget < 100 else 3), rounds.value
+    for k in ("steps", "end_state", "fate", "hops", "vcount"):
+        assert np.array_equal(getattr(got, k), getattr(want, k)), k
+    assert got.counters() == want.counters()
+    assert np.array_equal(got.density, want.density) and np.array_equal(got.horizon, want.horizon)
+    cnt = np.repeat(want.vcount.astype(np.float64), 3)
+    assert np.all(np.abs(got.vmom - want.vmom) <= 1e-12 * cnt * 5e3 + 1e-9)
+
+
+def test_velocity_records_match_oracle(lib, dev):
+    """Velocity records (ballistic.c:215-232) of 48 compat particles: with caps
+    that never bind, the device's records are the oracle's as a multiset (every
+    (vx, vy, vz, td, fd, ht) bit for bit; only the order of the first-come
+    appends differs); with the reference's caps (200 per (td, ht), 100 ftd hd
+    in all) each (td, ht) cell holds min(200, uncapped count) records, all of
+    them from that cell's uncapped records."""
+    T, ftd, ffd = read_T(GOLDEN / "T_thermal.dat")
+    p = compat_params(ftd, ffd, 48, 17)
+    surf = SurfaceArrays(T, ftd, ffd)
+    capped_cell, capped_total = p.record_cap_cell, p.record_cap_total
+    p.record_cap_cell, p.record_cap_total = 1 << 30, 1_500_000
+
+    def records(b):
+        n = int(b.result.vrec_n)
+        v = b.vrec[:3 * n].reshape(n, 3).view(np.int64)
+        rec = np.concatenate([v, np.stack([b.vrec_td[:n], b.vrec_fd[:n], b.vrec_ht[:n]], 1).astype(np.int64)], 1)
+        return rec[np.lexsort(rec.T[::-1])]
+
+    want = run_oracle(dev, p, surf, nthreads=8, vrec=True)
+    got = Buffers(p, 5, vrec=True)
+    print("velocity records: 48 compat particles, caps off", flush=True)
+    assert lib.dsmc_run_ballistic(C.byref(p), C.byref(surf.surface), C.byref(got.result)) == 0
+    assert 100_000 < want.result.vrec_n < p.record_cap_total
+    assert got.result.vrec_n == want.result.vrec_n
+    full = records(want)
+    assert np.array_equal(records(got), full)
+    cnt_full = np.bincount(full[:, 3] * p.hd + full[:, 5], minlength=ftd * p.hd)
+    assert cnt_full.max() > 200
+    assert np.minimum(cnt_full, capped_cell).sum() < capped_total      # the global cap does not bind
+    p.record_cap_cell, p.record_cap_total = capped_cell, capped_total
+    got = Buffers(p, 5, vrec=True)
+    print("velocity records: 48 compat particles, reference caps", flush=True)
+    assert lib.dsmc_run_ballistic(C.byref(p), C.byref(surf.surface), C.byref(got.result)) == 0
+    capped = records(got)
+    cnt = np.bincount(capped[:, 3] * p.hd + capped[:, 5], minlength=ftd * p.hd)
+    assert np.array_equal(cnt, np.minimum(cnt_full, capped_cell))
+    fullset = {tuple(r) for r in full}
+    assert all(tuple(r) in fullset for r in capped)
+
+
 def test_staged_run_relaunch(lib):
     """A staged run relaunched twice gives the same per-particle results."""
     p = capi.preset(lib, PRESET_CERES_PHYSICAL, 36, 72, n=20000, seed=3)
