@@ -21,6 +21,7 @@ import ctypes as C
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -60,22 +61,70 @@ def icosphere(lib, levels, lon=0.0, mass=2.99e-26, kB=1.380649e-23):
     return SurfaceArrays(levels=levels, vert=vert, faces=faces, face_T=fT, face_w=fw)
 
 
-def cpu_baseline(p, surf, sample: int, threads: int) -> dict:
-    """The oracle (libm build, OpenMP) on the first `sample` particles."""
+def host_cpus() -> dict:
+    """The host cores this process may use: its affinity set, the cgroup's CPU
+    quota (cpu.max, in whole cores), physical cores within the affinity set
+    (lscpu), and the CPU model."""
+    aff = sorted(os.sched_getaffinity(0))
+    info = {"affinity_cpus": len(aff), "quota_cpus": None, "physical_cores": None, "model": None}
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            info["quota_cpus"] = max(1, int(int(q) // int(per)))
+    except (OSError, ValueError):
+        pass
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, timeout=10).stdout
+        cores = {tuple(l.split(",")[1:3]) for l in out.splitlines()
+                 if l and not l.startswith("#") and int(l.split(",")[0]) in set(aff)}
+        info["physical_cores"] = len(cores) or None
+        m = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        info["model"] = next((l.split(":", 1)[1].strip() for l in m.splitlines() if l.startswith("Model name")),
+                             None)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    if info["model"] is None:
+        try:
+            info["model"] = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
+                                 if l.startswith("model name"))
+        except (OSError, StopIteration):
+            pass
+    return info
+
+
+def cpu_threads(info: dict) -> int:
+    """All the cores the process can run on at once: the physical cores of the
+    affinity set, bounded by the cgroup quota."""
+    n = info["physical_cores"] or info["affinity_cpus"]
+    if info["quota_cpus"]:
+        n = min(n, info["quota_cpus"])
+    return max(1, n)
+
+
+def cpu_baseline(p, surf, sample: int, threads: int, single_sample: int) -> dict:
+    """The oracle (libm build, OpenMP schedule(dynamic,100)) on the first `sample`
+    particles with `threads` threads, and on the first `single_sample` with one."""
     from tests.helpers import oracle
     lib = oracle(devmath=False)
-    q = p.copy()
-    q.nparticles = sample
-    b = Buffers(q, 5, per_particle=False, moments=True)
-    t0 = time.perf_counter()
-    rc = lib.cpu_ref_ballistic(C.byref(q), C.byref(surf.surface), C.byref(b.result), threads, 0)
-    dt = time.perf_counter() - t0
-    if rc:
-        raise RuntimeError(f"cpu_ref_ballistic: {rc}")
-    steps = b.result.total_steps
+
+    def timed(n: int, t: int):
+        q = p.copy()
+        q.nparticles = n
+        b = Buffers(q, 5, per_particle=False, moments=True)
+        t0 = time.perf_counter()
+        rc = lib.cpu_ref_ballistic(C.byref(q), C.byref(surf.surface), C.byref(b.result), t, 0)
+        dt = time.perf_counter() - t0
+        if rc:
+            raise RuntimeError(f"cpu_ref_ballistic: {rc}")
+        return b.result.total_steps, dt
+
+    steps, dt = timed(sample, threads)
+    s1, d1 = timed(single_sample, 1)
     return {"value": steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": f"first {sample} particles of the same workload ({steps} particle-steps, "
-                      f"{dt:.2f} s, oracle/libcpu_ref.so, OpenMP schedule(dynamic,100))"}
+                      f"{dt:.2f} s, oracle/libcpu_ref.so, OpenMP schedule(dynamic,100))",
+            "one_thread": {"value": s1 / d1, "sample": f"first {single_sample} particles ({s1} particle-steps, "
+                                                       f"{d1:.2f} s)"}}
 
 
 THERMAL_GRIDS = ((36, 72), (360, 720))
@@ -224,16 +273,16 @@ def plain_run(lib, surf, n, args, shape) -> dict:
                       ("n_escaped", "n_frozen", "n_photolysed", "n_capped")},
             "run_device_ms": [round(x, 3) for x in kms],
             "flight_kernel_ms": [round(x, 3) for x in fms], "rounds": rounds.value,
-            # compute-bound FP64: the push runs on the vector ALUs (no matrix
-            # cores); MI355X's FP64 vector and FP64 matrix peaks are both 78.6
-            "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+            # compute-bound FP64 on the vector ALUs (no matrix core is used):
+            # the peak is MI355X's FP64 vector rate, 78.6 TFLOP/s
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                          "compute": "FP64 vector ALU (VALU), not matrix cores",
                          "flops_per_particle_step": FLOPS_PER_STEP,
                          "flops_per_launch": FLOPS_PER_STEP * steps_per_run / launches,
                          "launches_per_run": launches,
                          "kernel": "phys_kernel<false>" if (shape is not None and shape.service_inline) else
-                         "flight_kernel<1,0>", "kernel_avg_ms_per_launch": favg * 1e3 / launches,
+                         "lean_flight_kernel", "kernel_avg_ms_per_launch": favg * 1e3 / launches,
                          "kernel_ms_per_run": favg * 1e3}}
 
 
@@ -263,12 +312,13 @@ def batch_once(lib, args, shape, params, surfs, moments: bool) -> dict:
     t_tune = time.perf_counter() - t_tune
     bufs = [Buffers(p, 5, per_particle=False, moments=moments) for p in params]
     arr = (C.POINTER(capi.Result) * n)(*[C.pointer(x.result) for x in bufs])
-    for _ in range(args.warmup):
+    warm, reps = min(args.warmup, 1), max(1, min(args.steps, 3))     # each launch is a whole batch
+    for _ in range(warm):
         if lib.dsmc_batch_launch(b, arr):
             raise RuntimeError("batch warmup failed")
     done = np.zeros(n)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(reps):
         if lib.dsmc_batch_launch(b, arr):
             raise RuntimeError("batch launch failed")
     wall = time.perf_counter() - t0
@@ -276,8 +326,8 @@ def batch_once(lib, args, shape, params, surfs, moments: bool) -> dict:
     lib.dsmc_batch_timing(b, C.byref(w), capi.ptr(done, C.c_double), C.byref(rounds), C.byref(used))
     lib.dsmc_batch_destroy(b)
     steps = sum(int(x.result.total_steps) for x in bufs)
-    return {"value": steps * args.steps / wall, "particle_steps_per_batch": steps,
-            "ms_per_batch": wall / args.steps * 1e3, "rounds": int(rounds.value),
+    return {"value": steps * reps / wall, "particle_steps_per_batch": steps, "launches_timed": reps,
+            "ms_per_batch": wall / reps * 1e3, "rounds": int(rounds.value),
             "run_finish_ms": percentiles(done), "autotune_s": round(t_tune, 3),
             "shape": {"block": used.block, "blocks_per_cu": used.blocks_per_cu,
                       "relaunch_batch": used.relaunch_batch, "step_budget": used.step_budget,
@@ -347,12 +397,15 @@ def collisional_run(lib, args) -> dict:
                              ("n_escaped", "n_frozen", "n_photolysed", "n_capped")}}
 
 
-def served_run(lib, args, shape, rate: float, seconds: float) -> dict:
+def served_run(lib, args, shape, rate: float, seconds: float, drain: bool = True) -> dict:
     """Poisson request stream (`rate` req/s for `seconds`) of runs, counts
     log-uniform on [1e4, 4e6], body and subsolar longitude drawn per request, 30%
     collisional (phase 1 + phase 2 at the request's count), each returning its
     40x36x72 density grid, through the continuous-batching server; latency =
-    arrival to result ready."""
+    arrival to result ready.  drain: serve until every request completed;
+    otherwise stop when the arrival window closes and report what completed in
+    it, the queue's growth, and latencies with the unfinished requests counted at
+    their age when the window closed (a lower bound)."""
     stream = W.request_stream(rate, seconds, seed=5, scale=args.serve_scale,
                               frac_collisional=args.serve_collisional)
     specs = [s for _, s in stream]
@@ -371,11 +424,16 @@ def served_run(lib, args, shape, rate: float, seconds: float) -> dict:
     i, outstanding = 0, set()
     first = True
     t_report = 0.0
+    growth = []
     while i < len(stream) or outstanding:
         if time.perf_counter() - t0 > t_report + 5.0:
             t_report = time.perf_counter() - t0
+            growth.append({"t_s": round(t_report, 1), "submitted": i, "outstanding": len(outstanding),
+                           "completed": i - len(outstanding)})
             progress(f"served: {i}/{len(stream)} submitted, {len(outstanding)} outstanding at {t_report:.0f} s")
         now = time.perf_counter() - t0
+        if not drain and now >= seconds:
+            break
         while i < len(stream) and stream[i][0] <= now:
             tk = C.c_uint64()
             if p2s[i] is not None:
@@ -408,18 +466,32 @@ def served_run(lib, args, shape, rate: float, seconds: float) -> dict:
     t_end = time.perf_counter() - t0
     stats = capi.ServerStats()
     lib.dsmc_server_get_stats(sv, C.byref(stats))
-    lib.dsmc_server_destroy(sv)
-    lat = [bufs[j].result.elapsed_ms + (t_sub[j] - stream[j][0]) * 1e3 for j in range(len(stream))]
+    unfinished = sorted(outstanding)
+    lib.dsmc_server_destroy(sv)         # waits for the replay in flight; unfinished requests are dropped
+    done = [j for j in range(i) if j not in outstanding]
+    lat = [bufs[j].result.elapsed_ms + (t_sub[j] - stream[j][0]) * 1e3 for j in done]
     steps = int(stats.particle_steps)
     ncoll = sum(1 for s in specs if s.collisional)
-    # offered load: the stream's whole work over the arrival window; above the
-    # served rate the queue grows for the whole window and latency with it
-    return {"value": steps / t_end, "unit": "particle-steps/s", "requests": len(stream),
-            "collisional_requests": ncoll, "offered_seconds": seconds,
-            "offered_steps_per_s": steps / seconds,
-            "rate_per_s": rate, "wall_s": t_end, "particles": int(sum(s.n for s in specs)),
-            "particle_steps": steps, "latency": percentiles(lat), "rounds": int(stats.rounds),
-            "slots": int(stats.slots)}
+    out = {"value": steps / t_end, "unit": "particle-steps/s", "requests": len(stream),
+           "collisional_requests": ncoll, "offered_seconds": seconds, "rate_per_s": rate, "wall_s": t_end,
+           "particles": int(sum(s.n for s in specs)), "particle_steps": steps, "latency": percentiles(lat),
+           "rounds": int(stats.rounds), "slots": int(stats.slots), "queue_growth": growth}
+    if drain:
+        # offered load: the stream's whole work over the arrival window; above the
+        # served rate the queue grows for the whole window and latency with it
+        out["offered_steps_per_s"] = steps / seconds
+    else:
+        # the work of the requests completed in the window, their latency, and
+        # every request's latency with the unfinished ones at their age at the
+        # window's end (a lower bound on what they would take)
+        ages = lat + [(t_end - stream[j][0]) * 1e3 for j in unfinished] + \
+               [(t_end - stream[j][0]) * 1e3 for j in range(i, len(stream)) if stream[j][0] <= t_end]
+        out.update({"completed": len(done), "unfinished_at_window_end": len(unfinished),
+                    "latency_completed": out.pop("latency"),
+                    "latency_all_lower_bound": percentiles(ages),
+                    "value_note": "particle-steps of the requests completed in the window / window length; "
+                                  "work done on unfinished requests is not counted"})
+    return out
 
 
 def dropin_run(lib, args) -> dict:
@@ -508,7 +580,12 @@ def main():
     ap.add_argument("--relaunch-batch", type=int, default=0)
     ap.add_argument("--step-budget", type=int, default=0)
     ap.add_argument("--service-inline", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=1500000, help="particles of the CPU baseline sample (~15 s)")
+    ap.add_argument("--late-step-budget", type=int, default=0, help="explicit shape: budget after two rounds")
+    ap.add_argument("--park-batch", type=int, default=0, help="explicit shape: parked lanes per cell-logic pass")
+    ap.add_argument("--cpu-sample", type=int, default=1500000,
+                    help="particles of the all-core CPU baseline sample (~15 s on 16 cores)")
+    ap.add_argument("--cpu-single-sample", type=int, default=60000,
+                    help="particles of the one-thread CPU baseline sample (~10 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-moments", action="store_true", help="density only (no velocity moments)")
     ap.add_argument("--no-batch", action="store_true")
@@ -518,7 +595,7 @@ def main():
     ap.add_argument("--batch-moments", type=int, default=1, help="also time the batch with velocity moments")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=20.0)
-    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-seconds", type=float, default=120.0)
     ap.add_argument("--serve-sustained-rate", type=float, default=2.0, help="second served leg (0: none)")
     ap.add_argument("--serve-sustained-seconds", type=float, default=10.0)
     ap.add_argument("--serve-scale", type=float, default=1.0)
@@ -543,9 +620,11 @@ def main():
     lib = capi.load()
     surf = icosphere(lib, 3)
     shape = None
-    if args.block or args.blocks_per_cu or args.relaunch_batch or args.step_budget or args.service_inline:
+    if (args.block or args.blocks_per_cu or args.relaunch_batch or args.step_budget or args.service_inline or
+            args.late_step_budget or args.park_batch):
         shape = capi.LaunchShape(args.block or 256, args.blocks_per_cu or 4, args.relaunch_batch or 16,
-                                 args.step_budget or 1024, args.service_inline)
+                                 args.step_budget or 1024, args.service_inline, args.late_step_budget,
+                                 args.park_batch)
     progress(f"plain run at {args.particles:.0e} particles")
     main_run = plain_run(lib, surf, args.particles, args, shape)
     progress(f"plain: {main_run['value']:.3e} particle-steps/s")
@@ -593,9 +672,10 @@ def main():
         progress(f"collisional: {line['collisional']['value']:.3e} particle-steps/s")
     if not args.no_serve:
         sshape = shape if shape is not None else tuned
-        progress(f"served: {args.serve_rate} req/s for {args.serve_seconds} s")
-        line["served"] = served_run(lib, args, sshape, args.serve_rate, args.serve_seconds)
-        progress(f"served: {line['served']['value']:.3e} particle-steps/s")
+        progress(f"served (Poisson stream): {args.serve_rate} req/s for {args.serve_seconds} s")
+        line["served_cfg5"] = served_run(lib, args, sshape, args.serve_rate, args.serve_seconds, drain=False)
+        progress(f"served: {line['served_cfg5']['value']:.3e} particle-steps/s, "
+                 f"{line['served_cfg5']['completed']}/{line['served_cfg5']['requests']} completed in the window")
         if args.serve_sustained_rate > 0:
             # the same stream at a rate the device keeps up with: latency is
             # then service time, not time spent queued behind a backlog
@@ -606,8 +686,10 @@ def main():
             progress(f"served (sustainable load): p50 {line['served_sustained']['latency']['p50_ms']:.0f} ms")
     if not args.no_cpu:
         progress("CPU baseline")
-        threads = min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(p, surf, args.cpu_sample, threads)
+        host = host_cpus()
+        threads = cpu_threads(host)
+        line["cpu_baseline"] = cpu_baseline(p, surf, args.cpu_sample, threads, args.cpu_single_sample)
+        line["cpu_baseline"]["host"] = host
         if not args.no_thermal:
             line["cpu_baseline"]["thermal"] = thermal_cpu(threads)
         if not args.no_shadow:

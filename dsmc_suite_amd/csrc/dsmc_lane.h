@@ -1095,6 +1095,39 @@ DSMC_HD void lane_step_compat(DsmcLane &L, const DsmcEnv &E, const DsmcCellTabs 
                               double nz, double Rn, double ds);
 
 /* ------------------------------------------------------------------ one flight step */
+/* A physical step after its push (the landing, escape, photolysis, cell, deposit
+ * and cap logic of lane_step), with the current cell's table entries CP read
+ * before the push.  Returns with L.phase = FLIGHT, LANDED, ENDED or COLLIDE. */
+template <class Ops, int ADV, class EnvT>
+DSMC_HD void lane_tail_phys(DsmcLane &L, const EnvT &E, const DsmcCellTabs &TB, const CellPre &CP, double Rn,
+                            double ds)
+{
+    const double nx = L.sx, ny = L.sy, nz = L.sz;
+    if (Rn <= E.R) { L.phase = LANE_LANDED; return; }
+    if (Rn >= E.max_height) { lane_flush<Ops>(L, E); L.fate = 1; L.phase = LANE_ENDED; return; }
+    if (nx * E.sun_x + ny * E.sun_y > 0.0) {
+        L.n_day++;
+        if (L.n_day >= L.n_photo) { lane_flush<Ops>(L, E); L.fate = 3; L.phase = LANE_ENDED; return; }
+    }
+    L.td = td_update_pre(nz, Rn, L.td, E.ftd, TB.tq, CP);
+    L.fd = fd_update_pre(nx, ny, L.fd, E.ffd, TB.cs, CP);
+    L.ht = table_walk_pre(TB.hr, L.ht, E.hd, Rn, CP);
+    if (L.ht < E.hd) {
+        const int cell = (L.ht * E.ftd + L.td) * E.ffd + L.fd;
+        if constexpr (ADV != 0) {
+            if (cell != L.ncell) { L.ncur = E.nsig[cell]; L.ncell = cell; }
+        }
+        lane_deposit<Ops, false>(L, E, cell);
+    } else if constexpr (ADV != 0) {
+        L.ncur = 0.0;
+        L.ncell = -1;
+    }
+    if constexpr (ADV != 0) L.real_col_p += L.ncur * ds;     /* collision depth */
+    int64_t cap = E.max_steps > 0 ? E.max_steps : INT64_MAX;
+    if (L.steps >= cap) { lane_flush<Ops>(L, E); L.fate = 5; L.phase = LANE_ENDED; }
+    else if (ADV != 0 && L.real_col_p >= L.col_p) L.phase = LANE_COLLIDE;
+}
+
 /* integrator.c:4-55 followed by ballistic.c:195-236 / advanced.c:347-402, or the
  * physical-mode step.  Returns with L.phase = FLIGHT, LANDED or DONE. */
 template <class Ops, int MODE, int ADV, class EnvT>
@@ -1108,36 +1141,72 @@ DSMC_HD void lane_step(DsmcLane &L, const EnvT &E, const DsmcCellTabs &TB)
     double ds = 0.0;
     const double Rn = verlet_push<ADV != 0>(L.sx, L.sy, L.sz, L.vx, L.vy, L.vz, L.ox, L.oy, L.oz, L.Ri, E.GM, h,
                                             E.c1, E.c2, &ds);
-    const double nx = L.sx, ny = L.sy, nz = L.sz;
     L.Ri = Rn;
-    if (Rn <= E.R) { L.phase = LANE_LANDED; return; }
-    if (Rn >= E.max_height) { lane_flush<Ops>(L, E); L.fate = 1; L.phase = LANE_ENDED; return; }
-
     if constexpr (MODE != 0) {
-        if (nx * E.sun_x + ny * E.sun_y > 0.0) {
-            L.n_day++;
-            if (L.n_day >= L.n_photo) { lane_flush<Ops>(L, E); L.fate = 3; L.phase = LANE_ENDED; return; }
-        }
-        L.td = td_update_pre(nz, Rn, L.td, E.ftd, TB.tq, CP);
-        L.fd = fd_update_pre(nx, ny, L.fd, E.ffd, TB.cs, CP);
-        L.ht = table_walk_pre(TB.hr, L.ht, E.hd, Rn, CP);
-        if (L.ht < E.hd) {
-            const int cell = (L.ht * E.ftd + L.td) * E.ffd + L.fd;
-            if constexpr (ADV != 0) {
-                if (cell != L.ncell) { L.ncur = E.nsig[cell]; L.ncell = cell; }
-            }
-            lane_deposit<Ops, false>(L, E, cell);
-        } else if constexpr (ADV != 0) {
-            L.ncur = 0.0;
-            L.ncell = -1;
-        }
-        if constexpr (ADV != 0) L.real_col_p += L.ncur * ds;     /* collision depth */
-        int64_t cap = E.max_steps > 0 ? E.max_steps : INT64_MAX;
-        if (L.steps >= cap) { lane_flush<Ops>(L, E); L.fate = 5; L.phase = LANE_ENDED; }
-        else if (ADV != 0 && L.real_col_p >= L.col_p) L.phase = LANE_COLLIDE;
+        lane_tail_phys<Ops, ADV>(L, E, TB, CP, Rn, ds);
     } else {
+        const double nx = L.sx, ny = L.sy, nz = L.sz;
+        if (Rn <= E.R) { L.phase = LANE_LANDED; return; }
+        if (Rn >= E.max_height) { lane_flush<Ops>(L, E); L.fate = 1; L.phase = LANE_ENDED; return; }
         lane_step_compat<Ops, ADV>(L, E, TB, nx, ny, nz, Rn, ds);
     }
+}
+
+/* ------------------------------------------------------------------ lean physical step */
+/* What a lane of the event-batched flight loops holds besides DsmcLane: the
+ * current cell's table entries (in registers, so a step reads no table), the
+ * step at which the lane must stop (the step cap or the launch's budget,
+ * whichever comes first) and whether a step may accumulate into the open visit
+ * without the cell logic: the cell is above the grid (no deposit), or it is
+ * the open visit's cell. */
+struct LeanLane {
+    CellPre B;
+    int64_t stop;
+    bool easy;      /* the next steps may take lane_fast_phys's shortcut */
+    bool pend;      /* a pushed step waits for lane_tail_phys */
+};
+
+/* Refreshes the cell entries and the shortcut flag after the cell logic ran. */
+template <class EnvT>
+DSMC_HD void lean_refresh(LeanLane &N, const DsmcLane &L, const EnvT &E, const DsmcCellTabs &TB)
+{
+    N.B = cell_pre(L.td, L.fd, L.ht, TB.tq, TB.cs, TB.hr);
+    N.easy = L.ht >= E.hd || L.acc_cell == (L.ht * E.ftd + L.td) * E.ffd + L.fd;
+}
+
+/* One physical step (ballistic, density and moments by visit) whose outcome is,
+ * in most steps, decided by a few comparisons: the push, then - when the point
+ * is firmly inside the current cell (the first tests of td_update_pre,
+ * fd_update_pre and table_walk_pre, on the entries the lane holds), neither
+ * landed nor escaped, not photolysed on this step, short of the stop step, and
+ * the open visit is this cell's - the sunlit count and the visit's sums, exactly
+ * as lane_tail_phys would apply them, and true.  Otherwise only the push is
+ * applied and it returns false: lane_tail_phys(L, ..., N.B, L.Ri, 0) completes
+ * the step. */
+template <class EnvT>
+DSMC_HD bool lane_fast_phys(DsmcLane &L, const EnvT &E, const LeanLane &N)
+{
+    L.steps++;
+    const double Rn = verlet_push<false>(L.sx, L.sy, L.sz, L.vx, L.vy, L.vz, L.ox, L.oy, L.oz, L.Ri, E.GM, E.h,
+                                         E.c1, E.c2, nullptr);
+    L.Ri = Rn;
+    const double x = L.sx, y = L.sy, z = L.sz;
+    const bool sun = x * E.sun_x + y * E.sun_y > 0.0;
+    const double et = 1e-12 * Rn;
+    const double ef = DSMC_FD_MARGIN * (fabs(x) + fabs(y));
+    const CellPre &B = N.B;
+    const bool stay = (z - B.tlo * Rn > et) & (B.thi * Rn - z > et) & (x * B.c0 - y * B.s0 > ef) &
+                      (x * B.c1 - y * B.s1 < -ef) & (Rn >= B.rlo) & (Rn < B.rhi);
+    const bool ok = stay & (Rn > E.R) & (Rn < E.max_height) & (L.steps < N.stop) & N.easy &
+                    !(sun & (L.n_day + 1 >= L.n_photo));
+    if (ok) {
+        L.n_day += sun ? 1 : 0;
+        if (L.ht < E.hd) {
+            L.acx += L.vx; L.acy += L.vy; L.acz += L.vz;
+            L.acc_n++;
+        }
+    }
+    return ok;
 }
 
 /* The rest of a compat step (ballistic.c:195-236 / advanced.c:347-402). */

@@ -289,7 +289,7 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
                 if (timed) HIP_OK(hipEventRecord(ev.back(), st));
             } else {
                 HIP_OK(dsmc_launch_flight(&rd->E, rec, Q, p, flight_grid, sh.block, sh.relaunch_batch,
-                                          issued < 2 ? sh_budget : late_budget, st));
+                                          issued < 2 ? sh_budget : late_budget, sh.park_batch, st));
                 if (timed) HIP_OK(hipEventRecord(ev.back(), st));
                 HIP_OK(dsmc_launch_service(&rd->E, rec, Q, p, service_grid, st));
             }
@@ -319,6 +319,7 @@ dsmc_launch_shape default_shape(const DsmcEnv &E, int short_hops)
     dsmc_launch_shape s;
     s.block = 256;
     s.late_step_budget = 0;
+    s.park_batch = 0;
     if (E.mode == 0 && !E.advanced) {
         /* compat ballistic particles make hundreds of hops, many of one step:
          * with the service in its own kernel every hop is a round of launches
@@ -371,7 +372,7 @@ bool shape_ok(const dsmc_launch_shape &s)
     return s.block >= 64 && s.block <= 256 && s.block % 64 == 0 && s.blocks_per_cu > 0 &&
            s.blocks_per_cu * s.block <= 2048 && s.relaunch_batch >= 1 && s.relaunch_batch <= 64 &&
            s.step_budget >= 16 && (s.service_inline == 0 || s.service_inline == 1) &&
-           (s.late_step_budget == 0 || s.late_step_budget >= 16);
+           (s.late_step_budget == 0 || s.late_step_budget >= 16) && s.park_batch >= 0 && s.park_batch <= 64;
 }
 
 int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st)

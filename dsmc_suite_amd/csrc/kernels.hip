@@ -294,6 +294,109 @@ flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch, in
     dep_merge(E);
 }
 
+/* Physical ballistic flights (the default path of large single runs), event-
+ * batched.  Every lane holds its cell's table entries in registers and takes
+ * lean steps (lane_fast_phys: the push and a few comparisons) while nothing but
+ * its step count and open visit change.  A step that needs more - a cell
+ * boundary within the margins, a landing, escape, photolysis, the step cap or
+ * the launch's budget, or the first deposit of a visit - parks the lane with its
+ * push done; once `pend_batch` lanes of the wavefront are parked (or every
+ * flying lane is), they run the rest of their step together (lane_tail_phys,
+ * the code every lane of the wavefront used to pass through on almost every
+ * step, since some lane crosses a cell boundary on most of them) and reload
+ * their cell's entries.  Same particles, same steps, same deposits as
+ * flight_kernel<1, 0>: a parked lane only waits. */
+#define LEAN_STEPS 2
+#define LEAN_PARK_BATCH 4
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch, int64_t step_budget,
+                   int pend_batch)
+{
+    extern __shared__ double lds_tabs[];
+    DsmcCellTabs TB;
+    {
+        const int ntq = E.ftd + 1, nhr = E.hd + 2, ncs = 2 * (E.ffd + 1);
+        for (int i = threadIdx.x; i < ntq + nhr + ncs; i += blockDim.x)
+            lds_tabs[i] = i < ntq ? E.tq[i] : (i < ntq + nhr ? E.hr[i - ntq] : E.cs[i - ntq - nhr]);
+        TB.tq = lds_tabs;
+        TB.hr = lds_tabs + ntq;
+        TB.cs = lds_tabs + ntq + nhr;
+        dep_init();
+        __syncthreads();
+    }
+    const unsigned int nq = Q.ctl[2 * p];
+    unsigned int *head = &Q.ctl[2 * p + 1];
+    const int32_t *flyq = Q.fly[p];
+    const uint32_t lane = __lane_id();
+    const int64_t cap = E.max_steps > 0 ? E.max_steps : INT64_MAX;
+    DsmcLane L;
+    L.phase = LANE_IDLE;
+    LeanLane N;
+    N.pend = false;
+    N.easy = false;
+    N.stop = 0;
+    int32_t slot = -1;
+    int64_t budget_end = 0;
+    bool exhausted = false;
+    for (;;) {
+        uint64_t idle = __ballot(L.phase == LANE_IDLE);
+        uint64_t flying = ~idle & __ballot(true);
+        if (!exhausted && idle && (__popcll(idle) >= refill_batch || flying == 0)) {
+            int leader = __ffsll((long long)idle) - 1;
+            unsigned int base = 0;
+            if ((int)lane == leader) base = atomicAdd(head, (unsigned int)__popcll(idle));
+            base = __shfl(base, leader);
+            if (base + (unsigned int)__popcll(idle) >= nq) exhausted = true;
+            if (idle & (1ull << lane)) {
+                unsigned int qi = base + __popcll(idle & lanemask_lt());
+                if (qi < nq) {
+                    slot = flyq[qi];
+                    rec_load_flight<1, 0>(L, rec[slot]);
+                    budget_end = step_budget > 0 ? L.steps + step_budget : INT64_MAX;
+                    N.stop = budget_end < cap ? budget_end : cap;
+                    N.pend = false;
+                    lean_refresh(N, L, E, TB);
+                }
+            }
+            flying = __ballot(L.phase == LANE_FLIGHT);
+        }
+        if (flying == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        for (;;) {
+#pragma unroll
+            for (int u = 0; u < LEAN_STEPS; u++)
+                if (L.phase == LANE_FLIGHT && !N.pend) N.pend = !lane_fast_phys(L, E, N);
+            const uint64_t parked = __ballot(N.pend);
+            if (parked && (__popcll(parked) >= pend_batch || parked == __ballot(L.phase == LANE_FLIGHT))) {
+                if (N.pend) {
+                    lane_tail_phys<DsmcOpsLds, 0>(L, E, TB, N.B, L.Ri, 0.0);
+                    N.pend = false;
+                    if (L.phase == LANE_FLIGHT) {
+                        if (L.steps >= budget_end) L.phase = LANE_IDLE + 8;
+                        else lean_refresh(N, L, E, TB);
+                    }
+                }
+            }
+            uint64_t ended = __ballot(L.phase != LANE_FLIGHT && L.phase != LANE_IDLE);
+            if (ended && (__popcll(ended | idle) >= refill_batch || __ballot(L.phase == LANE_FLIGHT) == 0)) break;
+        }
+        bool hop_end = L.phase == LANE_LANDED || L.phase == LANE_ENDED;
+        bool paused = L.phase == LANE_IDLE + 8;
+        if (hop_end || paused) {
+            if (hop_end) lane_flush<DsmcOpsLds>(L, E);     /* a pause keeps the open visit */
+            if (paused) L.phase = LANE_FLIGHT;
+            rec_store_flight<1, 0>(L, rec[slot]);
+        }
+        wave_append(hop_end, slot, Q.svc, &Q.ctl[4]);
+        wave_append(paused, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
+        if (hop_end || paused) L.phase = LANE_IDLE;
+    }
+    __syncthreads();
+    dep_merge(E);
+}
+
 /* Flight kernel of a pool: as above, with each lane's run constants in registers
  * and every run's radial thresholds in LDS (deposits go straight to the run's
  * grid; the LDS slot table is for single runs). */
@@ -702,12 +805,16 @@ extern "C" hipError_t dsmc_launch_start(const DsmcEnv *E, int32_t run_id, DsmcRe
 }
 
 extern "C" hipError_t dsmc_launch_flight(const DsmcEnv *E, DsmcRec *rec, DsmcQueues Q, int p, int grid, int block,
-                                         int refill_batch, int64_t step_budget, hipStream_t st)
+                                         int refill_batch, int64_t step_budget, int park_batch, hipStream_t st)
 {
     if (!block_ok(block)) return hipErrorInvalidValue;
     const int mode = E->mode, adv = E->advanced;
     const size_t lds = sizeof(double) * (size_t)((E->ftd + 1) + (E->hd + 2) + 2 * (E->ffd + 1));
-    DISPATCH(flight_kernel, dim3(grid), dim3(block), lds, st, *E, rec, Q, p, refill_batch, step_budget);
+    if (mode == 1 && !adv)
+        hipLaunchKernelGGL(lean_flight_kernel, dim3(grid), dim3(block), lds, st, *E, rec, Q, p, refill_batch,
+                           step_budget, park_batch > 0 ? park_batch : LEAN_PARK_BATCH);
+    else
+        DISPATCH(flight_kernel, dim3(grid), dim3(block), lds, st, *E, rec, Q, p, refill_batch, step_budget);
     return hipGetLastError();
 }
 
@@ -727,7 +834,7 @@ extern "C" int dsmc_flight_occupancy(const DsmcEnv *E, int inline_svc, int block
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, compat_kernel, block, lds);
     } else if (mode == 0 && !adv) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flight_kernel<0, 0>, block, lds);
     else if (mode == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flight_kernel<0, 1>, block, lds);
-    else if (!adv) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flight_kernel<1, 0>, block, lds);
+    else if (!adv) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lean_flight_kernel, block, lds);
     else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flight_kernel<1, 1>, block, lds);
     return e == hipSuccess ? nb : 0;
 }

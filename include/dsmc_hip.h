@@ -219,6 +219,8 @@ typedef struct dsmc_launch_shape {
                                  compat advanced runs always by the separate kernel) */
     int late_step_budget;     /* single runs with the separate service kernel: the step budget of
                                  every round after the first two; 0 = step_budget (pools ignore it) */
+    int park_batch;           /* physical ballistic flights: lanes whose step needs the cell logic
+                                 wait until this many of a wavefront do (1-64); 0 = the default */
 } dsmc_launch_shape;
 
 /* Staged runs: inputs uploaded once and kept resident in device memory, the run

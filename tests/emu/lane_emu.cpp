@@ -70,6 +70,49 @@ static void run_one_split(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t 
     }
 }
 
+/* The event-batched physical flight kernel (lean_flight_kernel): lean steps
+ * until one needs the cell logic, then lane_tail_phys and a reload of the cell
+ * entries; records and the service kernel as the split path. */
+static void run_one_lean(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t budget)
+{
+    DsmcRec R;
+    memset(&R, 0x5A, sizeof R);
+    DsmcLane L;
+    garbage(L);
+    lane_start<DsmcOpsHost, 1, 0>(L, E, n, cnt);
+    if (L.phase != LANE_FLIGHT) return;
+    rec_store_all<1>(L, R, 0);
+    DsmcCellTabs TB = {E.tq, E.hr, E.cs};
+    const int64_t cap = E.max_steps > 0 ? E.max_steps : INT64_MAX;
+    for (;;) {
+        DsmcLane F;
+        garbage(F);
+        rec_load_flight<1, 0>(F, R);
+        const int64_t end = F.steps + budget;
+        LeanLane N;
+        N.stop = end < cap ? end : cap;
+        N.pend = false;
+        lean_refresh(N, F, E, TB);
+        bool paused = false;
+        while (F.phase == LANE_FLIGHT) {
+            if (lane_fast_phys(F, E, N)) continue;
+            lane_tail_phys<DsmcOpsHost, 0>(F, E, TB, N.B, F.Ri, 0.0);
+            if (F.phase != LANE_FLIGHT) break;
+            if (F.steps >= end) { paused = true; break; }
+            lean_refresh(N, F, E, TB);
+        }
+        if (!paused) lane_flush<DsmcOpsHost>(F, E);   /* pauses keep the visit */
+        rec_store_flight<1, 0>(F, R);
+        if (paused) continue;
+        DsmcLane S;
+        garbage(S);
+        rec_load_all<1>(S, R);
+        lane_service<DsmcOpsHost, 1, 0>(S, E, cnt);
+        if (S.phase == LANE_DONE) return;
+        rec_store_all<1>(S, R, 0);
+    }
+}
+
 template <int MODE, int ADV>
 static void run_one_inline(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t budget)
 {
@@ -159,11 +202,12 @@ extern "C" int emu_advanced(const dsmc_params *p, const dsmc_surface *s, const d
 
 /* path 0: one particle at a time straight through the state machine; 1: the
  * split flight / service kernels' record traffic; 2: the physical flight kernel
- * with the service inline.  `budget`: steps per flight launch (paths 1, 2). */
+ * with the service inline; 3: the event-batched physical flight kernel (lean
+ * steps) with the service kernel.  `budget`: steps per flight launch (paths 1-3). */
 extern "C" int emu_ballistic_path(const dsmc_params *p, const dsmc_surface *s, dsmc_result *r, int path,
                                   int64_t budget)
 {
-    if (path < 0 || path > 2 || (path && budget < 1)) return -1;
+    if (path < 0 || path > 3 || (path && budget < 1) || (path == 3 && p->mode != DSMC_MODE_PHYSICAL)) return -1;
     DsmcEnv E;
     dsmc::HostTables T;
     int rc = dsmc::build_env(p, s, 0, &E, &T);
@@ -208,6 +252,7 @@ extern "C" int emu_ballistic_path(const dsmc_params *p, const dsmc_surface *s, d
         else if (path == 1) run_one_split<1, 0>(E, n, cnt, budget);
         else if (path == 2 && p->mode == DSMC_MODE_COMPAT) run_one_inline<0, 0>(E, n, cnt, budget);
         else if (path == 2) run_one_inline<1, 0>(E, n, cnt, budget);
+        else if (path == 3) run_one_lean(E, n, cnt, budget);
         else if (p->mode == DSMC_MODE_COMPAT) run_one<0, 0>(L, E, n, cnt);
         else run_one<1, 0>(L, E, n, cnt);
     }
