@@ -193,6 +193,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     P, S, R = C.POINTER(Params), C.POINTER(Surface), C.POINTER(Result)
     lib.dsmc_params_preset.argtypes = [P, C.c_int, C.c_int, C.c_int]
     lib.dsmc_run_ballistic.argtypes = [P, S, R]
+    lib.dsmc_run_ballistic_noalloc.argtypes = [P, S, R]
     lib.dsmc_run_advanced.argtypes = [P, S, C.POINTER(Phase2Input), R]
     lib.dsmc_build_fits.argtypes = [P, C.POINTER(Phase2Input), _i32p, _dp, _dp, _dp, _i32p]
     lib.dsmc_run_batch.argtypes = [C.c_int, P, S, R, C.POINTER(LaunchShape)]

@@ -10,10 +10,26 @@
 
 namespace dsmc {
 
-/* Owning device allocation. */
+/* A fixed block of device memory handed out front to back, 256-byte aligned;
+ * nothing is allocated or freed. */
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    void *take(size_t n)
+    {
+        const size_t a = (used + 255) & ~(size_t)255;
+        const size_t len = (n + 255) & ~(size_t)255;
+        if (a > cap || len > cap - a) return nullptr;
+        used = a + len;
+        return base + a;
+    }
+};
+
+/* Device allocation: owned (hipMalloc / hipFree), or borrowed from an Arena. */
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -22,6 +38,10 @@ struct DevBuf {
     int alloc(size_t n);
     int upload(const void *src, size_t n, hipStream_t s);
     int zeros(size_t n, hipStream_t s);
+    /* the same three from an arena: no allocation, cleared by a fill kernel */
+    int take(Arena &a, size_t n);
+    int upload(Arena &a, const void *src, size_t n, hipStream_t s);
+    int zeros(Arena &a, size_t n, hipStream_t s);
 };
 
 /* A non-blocking stream and two timing events for one call, released on every
@@ -59,9 +79,13 @@ struct RunDev {
     /* particle records and slot queues */
     DevBuf rec, flyq0, flyq1, svcq, qctl, counters, rec_total, env_d;
     unsigned int *host_ctl = nullptr;     /* pinned readback of qctl */
+    Arena *arena = nullptr;               /* device buffers from this block instead of hipMalloc */
+    unsigned int ctl_local[16];           /* readback of qctl when nothing may be allocated */
+    bool ctl_pinned = false;
     int rounds = 0;
     double flight_ms = 0.0;
     int short_hops = 0;                   /* physical: landings every few dozen steps */
+    void drop_buffers();                  /* every buffer released: a reused RunDev starts empty */
     ~RunDev();
 };
 

@@ -81,6 +81,7 @@ struct HostTables {
     std::vector<int32_t> vt_count;
     std::vector<double> cdf, enorm, fnorm;
     std::vector<int32_t> fmap;      /* build_face_map, where a surface is uploaded */
+    std::vector<int32_t> fmap_lat;  /* build_face_map's lattice (kept, so a reused HostTables reallocates nothing) */
     int level_off[12];
     int levels = 0, nfine = 0, nface_all = 0;
 };
@@ -246,14 +247,17 @@ inline int face_map_grid(int nfine)
  * face the lattice misses, or a cell with more than DSMC_FMAP_K faces, costs
  * only the full search: locate_face_near tests every candidate exactly.
  * `E` holds the surface scalars; enorm / fnorm are read from T. */
-inline void build_face_map(const DsmcEnv &E, const HostTables &T, int G, std::vector<int32_t> &out)
+inline void build_face_map(const DsmcEnv &E, const HostTables &T, int G, std::vector<int32_t> &out,
+                           std::vector<int32_t> *lattice = nullptr)
 {
     const int S = 4, L = G * S + 1;
     DsmcEnv H = E;
     H.enorm = T.enorm.data();
     H.fnorm = T.fnorm.data();
     H.fmap = nullptr;
-    std::vector<int32_t> lat((size_t)L * L);
+    std::vector<int32_t> own;
+    std::vector<int32_t> &lat = lattice ? *lattice : own;
+    lat.assign((size_t)L * L, 0);
     out.assign((size_t)6 * G * G * DSMC_FMAP_K, -1);
     for (int cf = 0; cf < 6; cf++) {
         const double sg = (cf & 1) ? -1.0 : 1.0;

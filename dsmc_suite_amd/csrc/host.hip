@@ -15,6 +15,7 @@
 #include <limits.h>
 #include <vector>
 #include <mutex>
+#include <chrono>
 
 #include "../../include/dsmc_hip.h"
 #include "dsmc_tables.h"
@@ -37,9 +38,10 @@ namespace dsmc {
 DevBuf::~DevBuf() { release(); }
 void DevBuf::release()
 {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr && !borrowed) (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
+    borrowed = false;
 }
 int DevBuf::alloc(size_t n)
 {
@@ -61,6 +63,31 @@ int DevBuf::zeros(size_t n, hipStream_t s)
     int rc = alloc(n);
     if (rc || n == 0) return rc;
     HIP_OK(hipMemsetAsync(ptr, 0, n, s));
+    return DSMC_OK;
+}
+
+int DevBuf::take(Arena &a, size_t n)
+{
+    release();
+    if (n == 0) return DSMC_OK;
+    ptr = a.take(n);
+    if (!ptr) return DSMC_ERR_BUDGET;
+    borrowed = true;
+    bytes = n;
+    return DSMC_OK;
+}
+int DevBuf::upload(Arena &a, const void *src, size_t n, hipStream_t s)
+{
+    int rc = take(a, n);
+    if (rc || n == 0) return rc;
+    HIP_OK(hipMemcpyAsync(ptr, src, n, hipMemcpyHostToDevice, s));
+    return DSMC_OK;
+}
+int DevBuf::zeros(Arena &a, size_t n, hipStream_t s)
+{
+    int rc = take(a, n);
+    if (rc || n == 0) return rc;
+    HIP_OK(dsmc_launch_fill0(ptr, (n + 7) / 8, s));     /* the arena rounds blocks up to 256 bytes */
     return DSMC_OK;
 }
 
@@ -90,11 +117,18 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
 {
     int rc = build_env(p, s, advanced, &rd->E, &rd->T);
     if (rc) return rc;
+    /* device buffers: hipMalloc'ed, or taken from the run's arena (cleared by a kernel) */
+    Arena *ar = rd->arena;
+    auto up = [&](DevBuf &b, const void *src, size_t n, hipStream_t s_) {
+        return ar ? b.upload(*ar, src, n, s_) : b.upload(src, n, s_);
+    };
+    auto zz = [&](DevBuf &b, size_t n, hipStream_t s_) { return ar ? b.zeros(*ar, n, s_) : b.zeros(n, s_); };
+    auto al = [&](DevBuf &b, size_t n) { return ar ? b.take(*ar, n) : b.alloc(n); };
     DsmcEnv &E = rd->E;
     HostTables &T = rd->T;
     const size_t cells = (size_t)p->hd * p->ftd * p->ffd;
     const int64_t n = p->nparticles;
-#define UP(buf, vec) do { if ((rc = (buf).upload((vec).data(), (vec).size() * sizeof((vec)[0]), st))) return rc; } while (0)
+#define UP(buf, vec) do { if ((rc = up((buf), (vec).data(), (vec).size() * sizeof((vec)[0]), st))) return rc; } while (0)
     UP(rd->tq, T.tq);
     UP(rd->hr, T.hr);
     UP(rd->cs, T.cs);
@@ -104,7 +138,7 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
     if (p->mode == DSMC_MODE_COMPAT) {
         UP(rd->vt_speeds, T.vt_speeds);
         UP(rd->vt_count, T.vt_count);
-        if ((rc = rd->Tsurf.upload(s->T_latlon, sizeof(double) * p->ftd * p->ffd, st))) return rc;
+        if ((rc = up(rd->Tsurf, s->T_latlon, sizeof(double) * p->ftd * p->ffd, st))) return rc;
         E.vt_speeds = (const double *)rd->vt_speeds.ptr;
         E.vt_count = (const int32_t *)rd->vt_count.ptr;
         E.T = (const double *)rd->Tsurf.ptr;
@@ -113,11 +147,11 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
         UP(rd->enorm, T.enorm);
         UP(rd->fnorm, T.fnorm);
         E.fmap_g = face_map_grid(T.nfine);
-        build_face_map(E, T, E.fmap_g, T.fmap);
+        build_face_map(E, T, E.fmap_g, T.fmap, &T.fmap_lat);
         UP(rd->fmap, T.fmap);
-        if ((rc = rd->vert.upload(s->vert, sizeof(double) * 3 * s->nvert, st))) return rc;
-        if ((rc = rd->faces.upload(s->faces, sizeof(int32_t) * 3 * T.nface_all, st))) return rc;
-        if ((rc = rd->face_T.upload(s->face_T, sizeof(double) * T.nfine, st))) return rc;
+        if ((rc = up(rd->vert, s->vert, sizeof(double) * 3 * s->nvert, st))) return rc;
+        if ((rc = up(rd->faces, s->faces, sizeof(int32_t) * 3 * T.nface_all, st))) return rc;
+        if ((rc = up(rd->face_T, s->face_T, sizeof(double) * T.nfine, st))) return rc;
         E.cdf = (const double *)rd->cdf.ptr;
         E.enorm = (const double *)rd->enorm.ptr;
         E.fnorm = (const double *)rd->fnorm.ptr;
@@ -134,24 +168,24 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
         rd->short_hops = sqrt(2.0 * p->G * p->M / p->radius) > 2.0 * v_th ? 1 : 0;
     }
 #undef UP
-    if ((rc = rd->density.zeros(cells * sizeof(double), st))) return rc;
-    if ((rc = rd->horizon.zeros(sizeof(double) * p->ftd * p->ffd, st))) return rc;
+    if ((rc = zz(rd->density, cells * sizeof(double), st))) return rc;
+    if ((rc = zz(rd->horizon, sizeof(double) * p->ftd * p->ffd, st))) return rc;
     E.density = (double *)rd->density.ptr;
     E.horizon = (double *)rd->horizon.ptr;
     if (r->vmom) {
-        if ((rc = rd->vmom.zeros(cells * 3 * sizeof(double), st))) return rc;
+        if ((rc = zz(rd->vmom, cells * 3 * sizeof(double), st))) return rc;
         E.vmom = (double *)rd->vmom.ptr;
     }
     if (r->vcount || p->mode == DSMC_MODE_PHYSICAL) {
         /* physical particles keep W = 1, so the density is the deposit count: one
          * atomic per cell visit instead of two */
-        if ((rc = rd->vcount.zeros(cells * sizeof(unsigned long long), st))) return rc;
+        if ((rc = zz(rd->vcount, cells * sizeof(unsigned long long), st))) return rc;
         E.vcount = (unsigned long long *)rd->vcount.ptr;
         E.density_from_count = p->mode == DSMC_MODE_PHYSICAL;
     }
     if (r->vhist && p->mode == DSMC_MODE_PHYSICAL && !advanced) {
         if (p->vhist_bins <= 0 || p->vhist_bins > 64 || !(p->vhist_max > 0)) return DSMC_ERR_ARG;
-        if ((rc = rd->vhist.zeros(cells * p->vhist_bins * sizeof(unsigned long long), st))) return rc;
+        if ((rc = zz(rd->vhist, cells * p->vhist_bins * sizeof(unsigned long long), st))) return rc;
         E.vhist = (unsigned long long *)rd->vhist.ptr;
         E.vhist_bins = p->vhist_bins;
         E.vhist_inv = (double)p->vhist_bins / p->vhist_max;
@@ -159,31 +193,31 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
     int rows = advanced ? 9 : 5;
     E.ttime_rows = rows;
     if (r->ttime && p->mode == DSMC_MODE_COMPAT) {
-        if ((rc = rd->ttime.zeros(sizeof(double) * rows * n, st))) return rc;
+        if ((rc = zz(rd->ttime, sizeof(double) * rows * n, st))) return rc;
         E.ttime = (double *)rd->ttime.ptr;
     }
     if (r->end_state) {
-        if ((rc = rd->end_state.zeros(sizeof(double) * 6 * n, st))) return rc;
+        if ((rc = zz(rd->end_state, sizeof(double) * 6 * n, st))) return rc;
         E.end_state = (double *)rd->end_state.ptr;
     }
     if (r->steps) {
-        if ((rc = rd->steps.zeros(sizeof(int64_t) * n, st))) return rc;
+        if ((rc = zz(rd->steps, sizeof(int64_t) * n, st))) return rc;
         E.steps_out = (int64_t *)rd->steps.ptr;
     }
     if (r->fate) {
-        if ((rc = rd->fate.zeros(sizeof(int32_t) * n, st))) return rc;
+        if ((rc = zz(rd->fate, sizeof(int32_t) * n, st))) return rc;
         E.fate_out = (int32_t *)rd->fate.ptr;
     }
     if (r->hops) {
-        if ((rc = rd->hops.zeros(sizeof(int32_t) * n, st))) return rc;
+        if ((rc = zz(rd->hops, sizeof(int32_t) * n, st))) return rc;
         E.hops_out = (int32_t *)rd->hops.ptr;
     }
     if (!advanced && p->mode == DSMC_MODE_COMPAT && p->record_cap_cell > 0 && r->vrec &&
         r->vrec_td && r->vrec_fd && r->vrec_ht && p->record_cap_total > 0) {
         size_t cap = (size_t)p->record_cap_total;
-        if ((rc = rd->vrec.zeros(cap * 3 * sizeof(double), st))) return rc;
-        if ((rc = rd->vrec_i.zeros(cap * 3 * sizeof(int32_t), st))) return rc;
-        if ((rc = rd->count_nn.zeros(sizeof(int32_t) * p->ftd * p->hd, st))) return rc;
+        if ((rc = zz(rd->vrec, cap * 3 * sizeof(double), st))) return rc;
+        if ((rc = zz(rd->vrec_i, cap * 3 * sizeof(int32_t), st))) return rc;
+        if ((rc = zz(rd->count_nn, sizeof(int32_t) * p->ftd * p->hd, st))) return rc;
         E.vrec = (double *)rd->vrec.ptr;
         E.vrec_td = (int32_t *)rd->vrec_i.ptr;
         E.vrec_fd = E.vrec_td + cap;
@@ -192,27 +226,39 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
     } else {
         E.rec_cap_cell = 0;
     }
-    if ((rc = rd->rec_total.zeros(sizeof(unsigned long long), st))) return rc;
-    if ((rc = rd->counters.zeros(sizeof(unsigned long long) * 8, st))) return rc;
+    if ((rc = zz(rd->rec_total, sizeof(unsigned long long), st))) return rc;
+    if ((rc = zz(rd->counters, sizeof(unsigned long long) * 8, st))) return rc;
     E.rec_total = (unsigned long long *)rd->rec_total.ptr;
     E.counters = (unsigned long long *)rd->counters.ptr;
     size_t ns = (size_t)(n > 0 ? n : 1);
     if (n > INT32_MAX) return DSMC_ERR_ARG;
-    if ((rc = rd->rec.alloc(ns * sizeof(DsmcRec)))) return rc;
-    if ((rc = rd->flyq0.alloc(ns * sizeof(int32_t)))) return rc;
-    if ((rc = rd->flyq1.alloc(ns * sizeof(int32_t)))) return rc;
-    if ((rc = rd->svcq.alloc(ns * sizeof(int32_t)))) return rc;
-    if ((rc = rd->qctl.zeros(sizeof(unsigned int) * 16, st))) return rc;
-    if (!rd->host_ctl && hipHostMalloc((void **)&rd->host_ctl, sizeof(unsigned int) * 16, 0) != hipSuccess)
-        return DSMC_ERR_NOMEM;
+    if ((rc = al(rd->rec, ns * sizeof(DsmcRec)))) return rc;
+    if ((rc = al(rd->flyq0, ns * sizeof(int32_t)))) return rc;
+    if ((rc = al(rd->flyq1, ns * sizeof(int32_t)))) return rc;
+    if ((rc = al(rd->svcq, ns * sizeof(int32_t)))) return rc;
+    if ((rc = zz(rd->qctl, sizeof(unsigned int) * 16, st))) return rc;
+    if (ar) rd->host_ctl = rd->ctl_local;
+    else if (!rd->host_ctl) {
+        if (hipHostMalloc((void **)&rd->host_ctl, sizeof(unsigned int) * 16, 0) != hipSuccess) return DSMC_ERR_NOMEM;
+        rd->ctl_pinned = true;
+    }
     /* room for the environment in device memory (written by launch_run) */
-    if ((rc = rd->env_d.alloc(sizeof(DsmcEnv)))) return rc;
+    if ((rc = al(rd->env_d, sizeof(DsmcEnv)))) return rc;
     return DSMC_OK;
+}
+
+void RunDev::drop_buffers()
+{
+    DevBuf *all[] = {&tq, &hr, &cs, &vt_speeds, &vt_count, &Tsurf, &cdf, &enorm, &fnorm, &fmap, &vert, &faces,
+                     &face_T, &density, &horizon, &vmom, &vcount, &ttime, &end_state, &steps, &fate, &hops, &vrec,
+                     &vrec_i, &count_nn, &ctl, &atm, &count_num, &coeffs, &pmin, &pmax, &pvalid, &speeds, &nsig,
+                     &vhist, &rec, &flyq0, &flyq1, &svcq, &qctl, &counters, &rec_total, &env_d};
+    for (DevBuf *b : all) b->release();
 }
 
 RunDev::~RunDev()
 {
-    if (host_ctl) (void)hipHostFree(host_ctl);
+    if (host_ctl && ctl_pinned) (void)hipHostFree(host_ctl);
 }
 
 /* Runs a staged run to completion on stream st: the start kernel, then rounds of
@@ -245,7 +291,7 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
     Q.svc = (int32_t *)rd->svcq.ptr;
     Q.ctl = (unsigned int *)rd->qctl.ptr;
     DsmcRec *rec = (DsmcRec *)rd->rec.ptr;
-    HIP_OK(hipMemsetAsync(Q.ctl, 0, sizeof(unsigned int) * 16, st));
+    HIP_OK(dsmc_launch_fill0(Q.ctl, sizeof(unsigned int) * 16 / 8, st));     /* a kernel, not a memset */
     /* the run's environment as the kernels' service code reads it from device memory */
     HIP_OK(hipMemcpyAsync(rd->env_d.ptr, &rd->E, sizeof(DsmcEnv), hipMemcpyHostToDevice, st));
     rd->rounds = 0;
@@ -389,11 +435,10 @@ int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st
     if (r->vmom && rd->vmom.ptr) HIP_OK(hipMemcpyAsync(r->vmom, rd->vmom.ptr, cells * 24, hipMemcpyDeviceToHost, st));
     if (r->vcount && rd->vcount.ptr)
         HIP_OK(hipMemcpyAsync(r->vcount, rd->vcount.ptr, cells * 8, hipMemcpyDeviceToHost, st));
-    std::vector<unsigned long long> cnt_tmp;
-    if (rd->E.density_from_count && r->density && !r->vcount) {
-        cnt_tmp.resize(cells);
-        HIP_OK(hipMemcpyAsync(cnt_tmp.data(), rd->vcount.ptr, cells * 8, hipMemcpyDeviceToHost, st));
-    }
+    /* physical runs: the density is the deposit count, staged in the caller's
+     * density array (8 bytes per cell either way) when no count array is given */
+    if (rd->E.density_from_count && r->density && !r->vcount)
+        HIP_OK(hipMemcpyAsync(r->density, rd->vcount.ptr, cells * 8, hipMemcpyDeviceToHost, st));
     if (r->ttime) {
         if (rd->ttime.ptr)
             HIP_OK(hipMemcpyAsync(r->ttime, rd->ttime.ptr, rd->ttime.bytes, hipMemcpyDeviceToHost, st));
@@ -408,8 +453,12 @@ int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st
         HIP_OK(hipMemcpyAsync(r->vhist, rd->vhist.ptr, rd->vhist.bytes, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     if (rd->E.density_from_count && r->density) {
-        const int64_t *c = r->vcount ? r->vcount : (const int64_t *)cnt_tmp.data();
-        for (size_t i = 0; i < cells; i++) r->density[i] = (double)c[i];
+        for (size_t i = 0; i < cells; i++) {
+            int64_t c;
+            if (r->vcount) c = r->vcount[i];
+            else memcpy(&c, &r->density[i], sizeof c);
+            r->density[i] = (double)c;
+        }
     }
     r->vrec_n = 0;
     if (rd->E.vrec) {
@@ -556,6 +605,68 @@ extern "C" int dsmc_run_ballistic(const dsmc_params *p, const dsmc_surface *s, d
         HIP_OK(hipEventElapsedTime(&ms, e0, e1));
         r->elapsed_ms = ms;
     }
+    return rc;
+}
+
+/* ------------------------------------------------------------------ allocation-free runs */
+/* Every device buffer of dsmc_run_ballistic_noalloc comes from this block (one
+ * copy per device, from module load until the process ends).  Host code refers
+ * to it only through the address hipGetSymbolAddress returns for the current
+ * device: the array's name is not a device address on the host. */
+#define DSMC_NOALLOC_BYTES ((size_t)96 << 20)
+static __device__ __align__(256) unsigned char scratch[DSMC_NOALLOC_BYTES];
+
+enum { NOALLOC_MAX_DEVICES = 64 };
+static std::mutex noalloc_mu;                      /* the process's calls take turns */
+static void *noalloc_base[NOALLOC_MAX_DEVICES];    /* scratch's address on each device */
+
+/* scratch's address on the current device, looked up on the device's first call. */
+static int noalloc_scratch(void **out)
+{
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return DSMC_ERR_DEVICE;
+    if (!noalloc_base[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fprintf(stderr, "dsmc_hip: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+            return DSMC_ERR_DEVICE;
+        }
+        noalloc_base[dev] = p;
+    }
+    *out = noalloc_base[dev];
+    return DSMC_OK;
+}
+
+extern "C" int dsmc_run_ballistic_noalloc(const dsmc_params *p, const dsmc_surface *s, dsmc_result *r)
+{
+    if (!p || !s || !r || p->nparticles < 0) return DSMC_ERR_ARG;
+    int rc = ensure_device();
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(noalloc_mu);
+    void *base = nullptr;
+    if ((rc = noalloc_scratch(&base))) return rc;
+    /* one RunDev for every call: its host tables keep their storage, its device
+     * buffers are borrowed from the arena, its queue readback is pageable */
+    static RunDev rd;
+    rd.drop_buffers();                            /* nothing of the previous call's run carries over */
+    Arena ar;
+    ar.base = (char *)base;
+    ar.cap = DSMC_NOALLOC_BYTES;
+    rd.arena = &ar;
+    const hipStream_t st = nullptr;               /* the null stream: no stream or event is created */
+    const auto t0 = std::chrono::steady_clock::now();
+    rc = stage_run(p, s, r, 0, &rd, st);
+    if (rc == DSMC_ERR_BUDGET)
+        fprintf(stderr, "dsmc_hip: dsmc_run_ballistic_noalloc: a run of %lld particles and %d x %d x %d cells needs "
+                        "more than the %zu MiB of preallocated device memory\n", (long long)p->nparticles, p->hd,
+                p->ftd, p->ffd, DSMC_NOALLOC_BYTES >> 20);
+    if (!rc) rc = launch_run(&rd, nullptr, st);
+    if (!rc) rc = collect_run(&rd, p, r, st);
+    if (!rc) r->elapsed_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    rd.drop_buffers();
+    rd.arena = nullptr;
     return rc;
 }
 

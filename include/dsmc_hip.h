@@ -205,6 +205,23 @@ int dsmc_run_ballistic(const dsmc_params *p, const dsmc_surface *s, dsmc_result 
 int dsmc_run_advanced(const dsmc_params *p, const dsmc_surface *s,
                       const dsmc_phase2_input *in, dsmc_result *r);
 
+/* dsmc_run_ballistic without allocating anything: no hipMalloc, hipFree or
+ * hipHostMalloc, no stream or event is created, buffers are cleared by a kernel
+ * (no memset).  Every device buffer of the run (particle records and queues,
+ * tables, grids, per-particle outputs) is taken from one 96 MiB block of device
+ * memory that the library holds on each device from load until the process
+ * ends, and the work runs on the null stream; the host tables reuse the
+ * previous call's storage.  Same inputs, outputs and results as
+ * dsmc_run_ballistic.  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation).  A run that does not fit the block is rejected with
+ * DSMC_ERR_BUDGET and a message (at the 50 x 36 x 72 grid with velocity
+ * moments: about 240k particles with every per-particle output, 300k without).
+ * Every call on a device uses that device's one block, which keeps the last
+ * call's contents: calls overlapping on one GPU would race, so the library
+ * runs the calls of a process one at a time. */
+int dsmc_run_ballistic_noalloc(const dsmc_params *p, const dsmc_surface *s, dsmc_result *r);
+
 /* Launch shape of the persistent particle kernel. */
 typedef struct dsmc_launch_shape {
     int block;                /* threads per block of the flight kernel (64, 128, 192 or 256) */
