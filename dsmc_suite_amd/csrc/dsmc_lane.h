@@ -1068,12 +1068,14 @@ DSMC_HD double sample_speed(const double *cg, int deg, double min_v, double max_
 /* integrator.c:4-55: one Stormer-Verlet step of position s and velocity v under
  * the attraction GM / r^2, with the previous step's acceleration o (replaced by
  * this step's); Ri = |s|, c1 = 1.5 h^2, c2 = h^2 / 6.  DS: *ds = |s_new - s|.
- * Returns |s_new|.  The reference's operation order, no contraction. */
+ * Returns |s_new|.  The reference's operation order, no contraction; the
+ * division and square root are the IEEE ones (dm_div_pos, dm_sqrt_pos: GM,
+ * r^3 and |s|^2 of a body's exosphere lie far inside their range). */
 template <bool DS>
 DSMC_HD double verlet_push(double &sx, double &sy, double &sz, double &vx, double &vy, double &vz, double &ox,
                            double &oy, double &oz, double Ri, double GM, double h, double c1, double c2, double *ds)
 {
-    const double g = GM / (Ri * Ri * Ri);
+    const double g = dm_div_pos(GM, Ri * Ri * Ri);
     const double ax = g * sx, ay = g * sy, az = g * sz;
     const double nx = sx + vx * h - ax * c1 - ox * c2;
     const double ny = sy + vy * h - ay * c1 - oy * c2;
@@ -1087,7 +1089,7 @@ DSMC_HD double verlet_push(double &sx, double &sy, double &sz, double &vx, doubl
     vz -= az * h;
     ox = ax; oy = ay; oz = az;
     sx = nx; sy = ny; sz = nz;
-    return sqrt(nx * nx + ny * ny + nz * nz);
+    return dm_sqrt_pos(nx * nx + ny * ny + nz * nz);
 }
 
 template <class Ops, int ADV>

@@ -239,4 +239,46 @@ DM_FN double dm_log10(double x) { return dm_log(x) / DM_LN10; }
 DM_FN double dm_pow10(double y) { return dm_exp(y * DM_LN10); }
 DM_FN double dm_root4(double x) { return sqrt(sqrt(x)); }
 
+/* ---------------------------------------------------------------- a / b and sqrt(x), in range */
+/* The correctly rounded FP64 division and square root the compiler emits for
+ * gfx950 wrap the same reciprocal / reciprocal-square-root refinement in range
+ * scaling (v_div_scale, ldexp by 2^256 below 2^-767) and special-case fix-up
+ * (v_div_fixup, the class test for 0 and infinity).  For positive normal
+ * operands well inside the range - a quotient and operands between 2^-500 and
+ * 2^500 - the scaling is by 2^0 and the fix-up returns its input, so these
+ * refinements alone give the same bits with fewer dependent instructions.  On the
+ * host: the IEEE operations.  For the push: GM / r^3 and |s|^2. */
+DM_FN double dm_div_pos(double a, double b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    const double q = a * r;
+    const double rem = __builtin_fma(-b, q, a);
+    return __builtin_fma(rem, r, q);
+#else
+    return a / b;
+#endif
+}
+
+DM_FN double dm_sqrt_pos(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+#else
+    return sqrt(x);
+#endif
+}
+
 #endif /* DSMC_MATH_H */

@@ -745,17 +745,11 @@ __global__ void dsmc_verlet_kernel(double *state, int n, int nsteps, double GM, 
     double *s = state + (size_t)i * 9;
     double x = s[0], y = s[1], z = s[2], vx = s[3], vy = s[4], vz = s[5], ox = s[6], oy = s[7], oz = s[8];
     const double h2 = h * h, c1 = 1.5 * h2, c2 = h2 / 6.0;
+    double Ri = sqrt(x * x + y * y + z * z);
     for (int k = 0; k < nsteps; k++) {
-        double Ri = sqrt(x * x + y * y + z * z);
-        double g = GM / (Ri * Ri * Ri);
-        double ax = g * x, ay = g * y, az = g * z;
-        double nx = x + vx * h - ax * c1 - ox * c2;
-        double ny = y + vy * h - ay * c1 - oy * c2;
-        double nz = z + vz * h - az * c1 - oz * c2;
-        vx -= ax * h; vy -= ay * h; vz -= az * h;
-        ox = ax; oy = ay; oz = az;
-        x = nx; y = ny; z = nz;
-        double Rn = sqrt(x * x + y * y + z * z);
+        /* the kernels' own step (dm_div_pos, dm_sqrt_pos inside) */
+        const double Rn = verlet_push<false>(x, y, z, vx, vy, vz, ox, oy, oz, Ri, GM, h, c1, c2, nullptr);
+        Ri = Rn;
         double st = Rn <= R ? 1.0 : (Rn >= max_height ? 2.0 : 0.0);
         if (traj) {
             double *t = traj + ((size_t)i * nsteps + k) * 7;
@@ -766,7 +760,8 @@ __global__ void dsmc_verlet_kernel(double *state, int n, int nsteps, double GM, 
 }
 
 /* Elementary functions of dsmc_math.h on the device (bitwise host/device check):
- * fn 0 sin, 1 cos, 2 asin, 3 atan2(x,y), 4 exp, 5 log, 6 sqrt, 7 x/y. */
+ * fn 0 sin, 1 cos, 2 asin, 3 atan2(x,y), 4 exp, 5 log, 6 sqrt, 7 x/y,
+ * 8 dm_div_pos(x, y), 9 dm_sqrt_pos(x). */
 __global__ void dsmc_math_kernel(int fn, const double *x, const double *y, double *out, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -780,6 +775,8 @@ __global__ void dsmc_math_kernel(int fn, const double *x, const double *y, doubl
     case 4: r = dm_exp(a); break;
     case 5: r = dm_log(a); break;
     case 6: r = sqrt(a); break;
+    case 8: r = dm_div_pos(a, b); break;
+    case 9: r = dm_sqrt_pos(a); break;
     default: r = a / b; break;
     }
     out[i] = r;

@@ -259,7 +259,8 @@ void dsmc_run_destroy(dsmc_run *run);
 /* Diagnostics on the device: independent Verlet steps (integrator.c:4-55) of
  * n states [pos, vel, oa] x 9, optional trajectory [n][nsteps][pos, vel, status];
  * and the device elementary functions (0 sin, 1 cos, 2 asin, 3 atan2(x, y),
- * 4 exp, 5 log, 6 sqrt, 7 x / y). */
+ * 4 exp, 5 log, 6 sqrt, 7 x / y, 8 x / y and 9 sqrt(x) for positive operands in
+ * [2^-500, 2^500] as the push evaluates them). */
 int dsmc_verlet_steps(double *state, int n, int nsteps, double GM, double R, double max_height,
                       double h, double *traj);
 int dsmc_elementary(int fn, const double *x, const double *y, double *out, int n);

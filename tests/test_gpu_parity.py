@@ -65,6 +65,39 @@ def test_device_math_bitwise(lib, dev):
     assert np.array_equal(out, x / y)
 
 
+def test_push_division_and_sqrt_are_ieee(lib):
+    """The push's division and square root (dm_div_pos, dm_sqrt_pos: the
+    correctly rounded refinements without range scaling) equal IEEE division and
+    sqrt bit for bit: 4M operand pairs log-uniform over [2^-400, 2^400] (the
+    quotient within range too), the push's own ranges (GM 1e9-1e14 over r^3 of
+    R..20R for bodies of 100-3000 km; |s|^2 of the same radii), ulp-neighbours of
+    powers of two and of perfect squares."""
+    rng = np.random.default_rng(7)
+    n = 1 << 22
+    ea, eb = rng.uniform(-400, 400, n), rng.uniform(-400, 400, n)
+    keep = np.abs(ea - eb) < 400
+    a = np.exp2(ea[keep]) * rng.uniform(1, 2, keep.sum())
+    b = np.exp2(eb[keep]) * rng.uniform(1, 2, keep.sum())
+    gm = 10 ** rng.uniform(9, 14, 1 << 20)
+    r = 10 ** rng.uniform(5, 7.8, 1 << 20)
+    p2 = np.ldexp(1.0, np.arange(-400, 401))
+    sq = np.arange(1, 200001, dtype=np.float64) ** 2
+    edge = np.concatenate([p2, np.nextafter(p2, 0), np.nextafter(p2, np.inf), sq, np.nextafter(sq, 0),
+                           np.nextafter(sq, np.inf)])
+    for x, y in ((a, b), (gm, r ** 3), (edge, edge[::-1].copy())):
+        x, y = np.ascontiguousarray(x), np.ascontiguousarray(y)
+        m = len(x)
+        out = np.zeros(m)
+        print(f"dm_div_pos on {m} pairs", flush=True)
+        assert lib.dsmc_elementary(8, ptr(x, C.c_double), ptr(y, C.c_double), ptr(out, C.c_double), m) == 0
+        assert np.array_equal(out.view(np.int64), (x / y).view(np.int64))
+    for x in (a, r * r, edge):
+        x = np.ascontiguousarray(x)
+        out = np.zeros(len(x))
+        assert lib.dsmc_elementary(9, ptr(x, C.c_double), None, ptr(out, C.c_double), len(x)) == 0
+        assert np.array_equal(out.view(np.int64), np.sqrt(x).view(np.int64))
+
+
 def test_verlet_matches_reference_trace(lib):
     """The device integrator reproduces consecutive states of the reference's
     verlet_step trace (integrator.c:4-55) bit for bit."""
