@@ -72,7 +72,7 @@ struct RunDev {
     DsmcEnv E;
     HostTables T;
     DevBuf tq, hr, cs, vt_speeds, vt_count, Tsurf, cdf, enorm, fnorm, fmap, vert, faces, face_T;
-    DevBuf density, horizon, vmom, vcount, ttime, end_state, steps, fate, hops;
+    DevBuf density, horizon, vmom, vcount, cellacc, ttime, end_state, steps, fate, hops;
     DevBuf vrec, vrec_i, count_nn, ctl;
     /* advanced */
     DevBuf atm, count_num, coeffs, pmin, pmax, pvalid, speeds, nsig, vhist;

@@ -97,6 +97,10 @@ struct DsmcEnv {
     int density_from_count;   /* physical mode with counts: W == 1, density = vcount */
     double *density, *horizon, *vmom;
     unsigned long long *vcount;
+    /* physical runs with moments, lean flight kernel: per-cell records (sum vx,
+     * sum vy, sum vz, count) added by 4 lanes at once and folded into vmom and
+     * vcount after the run, [cells*4]; nullptr: deposits go to vmom/vcount */
+    double *cellacc;
     double *ttime, *end_state;
     int64_t *steps_out;
     int32_t *fate_out, *hops_out;

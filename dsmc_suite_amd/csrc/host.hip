@@ -175,6 +175,11 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
     if (r->vmom) {
         if ((rc = zz(rd->vmom, cells * 3 * sizeof(double), st))) return rc;
         E.vmom = (double *)rd->vmom.ptr;
+        /* physical ballistic runs: the lean flight kernel's per-cell records */
+        if (p->mode == DSMC_MODE_PHYSICAL && !advanced) {
+            if ((rc = zz(rd->cellacc, cells * 4 * sizeof(double), st))) return rc;
+            E.cellacc = (double *)rd->cellacc.ptr;
+        }
     }
     if (r->vcount || p->mode == DSMC_MODE_PHYSICAL) {
         /* physical particles keep W = 1, so the density is the deposit count: one
@@ -250,9 +255,9 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
 void RunDev::drop_buffers()
 {
     DevBuf *all[] = {&tq, &hr, &cs, &vt_speeds, &vt_count, &Tsurf, &cdf, &enorm, &fnorm, &fmap, &vert, &faces,
-                     &face_T, &density, &horizon, &vmom, &vcount, &ttime, &end_state, &steps, &fate, &hops, &vrec,
-                     &vrec_i, &count_nn, &ctl, &atm, &count_num, &coeffs, &pmin, &pmax, &pvalid, &speeds, &nsig,
-                     &vhist, &rec, &flyq0, &flyq1, &svcq, &qctl, &counters, &rec_total, &env_d};
+                     &face_T, &density, &horizon, &vmom, &vcount, &cellacc, &ttime, &end_state, &steps, &fate,
+                     &hops, &vrec, &vrec_i, &count_nn, &ctl, &atm, &count_num, &coeffs, &pmin, &pmax, &pvalid,
+                     &speeds, &nsig, &vhist, &rec, &flyq0, &flyq1, &svcq, &qctl, &counters, &rec_total, &env_d};
     for (DevBuf *b : all) b->release();
 }
 
@@ -351,6 +356,8 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
         }
         if (chunk < 128) chunk *= 2;
     }
+    if (!inline_svc && rd->E.mode == DSMC_MODE_PHYSICAL && !rd->E.advanced)
+        HIP_OK(dsmc_launch_fold(&rd->E, (int64_t)rd->E.hd * rd->E.ftd * rd->E.ffd, st));
     rd->rounds = (int)rd->host_ctl[8];
     for (size_t i = 0; i + 1 < ev.size(); i += 2) {
         float ms = 0.f;
@@ -701,7 +708,7 @@ extern "C" dsmc_run *dsmc_run_create(const dsmc_params *p, const dsmc_surface *s
 /* Clears every output buffer of a staged run on its stream. */
 static int clear_outputs(RunDev &rd, hipStream_t st)
 {
-    DevBuf *bufs[] = {&rd.density, &rd.horizon, &rd.vmom, &rd.vcount, &rd.ttime, &rd.end_state,
+    DevBuf *bufs[] = {&rd.density, &rd.horizon, &rd.vmom, &rd.vcount, &rd.cellacc, &rd.ttime, &rd.end_state,
                       &rd.steps, &rd.fate, &rd.hops, &rd.count_nn, &rd.counters, &rd.rec_total, &rd.vhist};
     for (DevBuf *b : bufs)
         if (b->ptr) HIP_OK(hipMemsetAsync(b->ptr, 0, b->bytes, st));
