@@ -216,7 +216,7 @@ int dsmc_run_advanced(const dsmc_params *p, const dsmc_surface *s,
  * cannot allocate (code under HIP graph capture, tight serving loops, harnesses
  * that forbid allocation).  A run that does not fit the block is rejected with
  * DSMC_ERR_BUDGET and a message (at the 50 x 36 x 72 grid with velocity
- * moments: about 280k particles with every per-particle output, 340k without).
+ * moments: about 270k particles with every per-particle output, 335k without).
  * Every call on a device uses that device's one block, which keeps the last
  * call's contents: calls overlapping on one GPU would race, so the library
  * runs the calls of a process one at a time. */
