@@ -36,8 +36,8 @@ from dsmc_suite_amd import workloads as W  # noqa: E402
 from dsmc_suite_amd.capi import PRESET_CERES_PHYSICAL, PRESET_COMPAT_BALLISTIC, Buffers, SurfaceArrays, ptr  # noqa: E402
 
 METRIC = "particle-steps/sec FP64, plain vs batched/served, 1 MI355X; p50/p99 run latency"
-# Algorithmic FP64 operations of one particle-step of the push (Verlet 51 +
-# cell/photolysis/deposit bookkeeping 15), counted as in the design notes.
+# Algorithmic FP64 operations of one particle-step: the Verlet push 51, the
+# cell/photolysis/deposit bookkeeping 15.
 FLOPS_PER_STEP = 66
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch of the flight kernel on the 1e7-particle Ceres run with
@@ -45,7 +45,7 @@ FP64_PEAK_TFLOPS = 78.6
 # (per dispatch, averaged over the run's launches), each scaled by what the
 # counter reads for a plain 1 GiB device-to-device copy (FETCH_SIZE x 2.00,
 # WRITE_SIZE x 1.00).  Only that configuration reports it; others give null.
-PMC_TRAFFIC_1E7 = 153_930_711 + 3_356_706_875      # read + written, bytes per launch
+PMC_TRAFFIC_1E7 = 206_567_454 + 1_046_551_722      # read + written, bytes per launch
 
 
 def icosphere(lib, levels, lon=0.0, mass=2.99e-26, kB=1.380649e-23):
