@@ -52,7 +52,7 @@ hipError_t dsmc_launch_hist_fits(const unsigned long long *vhist, int ncell, int
                                  int32_t *count_num, hipStream_t st);
 hipError_t dsmc_launch_nsig(const double *ndens, const int32_t *valid, double sigma, int ncell, double *nsig,
                             hipStream_t st);
-/* the lean flight kernel's per-cell records into vmom and vcount (no-op without records) */
+/* a physical run's per-cell records into vmom and vcount (no-op without records) */
 hipError_t dsmc_launch_fold(const DsmcEnv *E, int64_t cells, hipStream_t st);
 /* n zeros of 8 bytes (graph-safe clear) */
 hipError_t dsmc_launch_fill0(void *p, size_t n8, hipStream_t st);

@@ -97,9 +97,9 @@ struct DsmcEnv {
     int density_from_count;   /* physical mode with counts: W == 1, density = vcount */
     double *density, *horizon, *vmom;
     unsigned long long *vcount;
-    /* physical runs with moments, lean flight kernel: per-cell records (sum vx,
-     * sum vy, sum vz, count) added by 4 lanes at once and folded into vmom and
-     * vcount after the run, [cells*4]; nullptr: deposits go to vmom/vcount */
+    /* physical runs with moments: every deposit goes to a 32-byte record per
+     * cell (sum vx, sum vy, sum vz, count), [cells*4], folded into vmom and
+     * vcount when the run ends; nullptr: deposits go to vmom / vcount */
     double *cellacc;
     double *ttime, *end_state;
     int64_t *steps_out;
@@ -125,6 +125,7 @@ struct DsmcStepEnv {
     int ftd, ffd, hd, density_from_count;
     double *density, *vmom;
     unsigned long long *vcount;
+    double *cellacc;          /* moments runs: per-cell records (as DsmcEnv::cellacc) */
     int hr_off;               /* this run's radial thresholds within the pool's table */
     int pad_;
     const double *nsig;       /* collisional runs: n*sigma per cell */

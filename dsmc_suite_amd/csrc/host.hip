@@ -175,8 +175,8 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
     if (r->vmom) {
         if ((rc = zz(rd->vmom, cells * 3 * sizeof(double), st))) return rc;
         E.vmom = (double *)rd->vmom.ptr;
-        /* physical ballistic runs: the lean flight kernel's per-cell records */
-        if (p->mode == DSMC_MODE_PHYSICAL && !advanced) {
+        /* physical runs: deposits go to per-cell records (folded at the end) */
+        if (p->mode == DSMC_MODE_PHYSICAL) {
             if ((rc = zz(rd->cellacc, cells * 4 * sizeof(double), st))) return rc;
             E.cellacc = (double *)rd->cellacc.ptr;
         }
@@ -356,8 +356,8 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
         }
         if (chunk < 128) chunk *= 2;
     }
-    if (!inline_svc && rd->E.mode == DSMC_MODE_PHYSICAL && !rd->E.advanced)
-        HIP_OK(dsmc_launch_fold(&rd->E, (int64_t)rd->E.hd * rd->E.ftd * rd->E.ffd, st));
+    /* physical runs with moments deposited into per-cell records: into vmom, vcount */
+    HIP_OK(dsmc_launch_fold(&rd->E, (int64_t)rd->E.hd * rd->E.ftd * rd->E.ffd, st));
     rd->rounds = (int)rd->host_ctl[8];
     for (size_t i = 0; i + 1 < ev.size(); i += 2) {
         float ms = 0.f;

@@ -62,6 +62,11 @@ template <class EnvT>
 __device__ static inline void grid_deposit(const EnvT &E, int cell, double w, double vx, double vy,
                                            double vz, int n)
 {
+    if (E.cellacc) {      /* physical runs with moments: the cell's 32-byte record */
+        double *a = E.cellacc + 4 * (size_t)cell;
+        gadd(&a[0], vx); gadd(&a[1], vy); gadd(&a[2], vz); gadd(&a[3], (double)n);
+        return;
+    }
     if (!E.density_from_count) gadd(&E.density[cell], w);
     if (E.vmom) {
         gadd(&E.vmom[(size_t)cell * 3 + 0], vx);
@@ -160,42 +165,33 @@ __device__ static inline void wave_append(bool pred, int32_t slot, int32_t *q, u
     if (pred) q[base + __popcll(m & lanemask_lt())] = slot;
 }
 
-/* Staged deposits of the lean flight kernel (physical runs: weight 1, density =
- * count).  A device-scope atomic leaves the L2 as one request per lane, and one
- * request carries a 32-byte sector: four lanes adding the four words of one
- * cell's record (sum vx, sum vy, sum vz, count) in one instruction make one
- * request where one lane adding them in turn makes four (on MI355X, 129600
- * cells, random cells: 2.36e10 against 5.8e9 cell deposits/s).  A lane's
- * deposit goes to one of its own STAGE_SLOTS LDS slots; then, in uniform
- * control flow, the wavefront issues every staged visit, 16 cells per atomic
- * instruction (64 when the run keeps counts only and the record is not used). */
+/* Staged deposits of physical runs with velocity moments (weight 1, density =
+ * count, every deposit to the cell's 32-byte record).  A device-scope atomic
+ * leaves the L2 as one request per lane, and one request carries a 32-byte
+ * sector: four lanes adding the four words of one cell's record (sum vx, sum
+ * vy, sum vz, count) in one instruction make one request where one lane adding
+ * them in turn makes four (on MI355X, 129600 cells, random cells: 2.36e10
+ * against 5.8e9 cell deposits/s).  A lane's deposit goes to one of its own
+ * STAGE_SLOTS LDS slots (the record's address and the four values); then, in
+ * uniform control flow, the wavefront issues every staged deposit, 16 records
+ * per atomic instruction.  Runs without moments deposit one count per visit,
+ * one request either way: those go straight to the grid. */
 #define STAGE_SLOTS 2
 __shared__ double stg_val[256 * STAGE_SLOTS * 4];
-__shared__ int32_t stg_cell[256 * STAGE_SLOTS];
+__shared__ unsigned long long stg_ptr[256 * STAGE_SLOTS];
 __shared__ int32_t stg_cnt[256];
 __shared__ int32_t stg_idx[4][64 * STAGE_SLOTS];
 
-/* One visit straight to the grid: the record when the run has one, else the count. */
-__device__ static inline void cell_deposit(const DsmcEnv &E, int cell, double vx, double vy, double vz, int n)
-{
-    if (E.cellacc) {
-        double *a = E.cellacc + 4 * (size_t)cell;
-        gadd(&a[0], vx); gadd(&a[1], vy); gadd(&a[2], vz); gadd(&a[3], (double)n);
-    } else {
-        gadd(&E.vcount[cell], (unsigned long long)n);
-    }
-}
-
 struct DsmcOpsStage {
-    __device__ static inline void deposit(const DsmcEnv &E, int cell, double w, double vx, double vy, double vz,
+    template <class EnvT>
+    __device__ static inline void deposit(const EnvT &E, int cell, double w, double vx, double vy, double vz,
                                           int n)
     {
-        (void)w;
         visit_hist(E, cell, vx, vy, vz, n);
         const int t = threadIdx.x, k = stg_cnt[t];
-        if (k >= STAGE_SLOTS) { cell_deposit(E, cell, vx, vy, vz, n); return; }
+        if (!E.cellacc || k >= STAGE_SLOTS) { grid_deposit(E, cell, w, vx, vy, vz, n); return; }
         const int e = t * STAGE_SLOTS + k;
-        stg_cell[e] = cell;
+        stg_ptr[e] = (unsigned long long)(E.cellacc + 4 * (size_t)cell);
         stg_val[4 * e + 0] = vx; stg_val[4 * e + 1] = vy; stg_val[4 * e + 2] = vz; stg_val[4 * e + 3] = (double)n;
         stg_cnt[t] = k + 1;
     }
@@ -205,13 +201,17 @@ struct DsmcOpsStage {
     __device__ static inline unsigned long long claim_u64(unsigned long long *p) { return gclaim(p); }
 };
 
-/* Issues the wavefront's staged visits; every lane of the wavefront calls it. */
-__device__ static inline void stage_issue(const DsmcEnv &E)
+__device__ static inline void stage_init() { stg_cnt[threadIdx.x] = 0; }
+
+/* Issues the wavefront's staged deposits; every lane of the wavefront calls it.
+ * List: slot 0 of every lane that has one, then slot 1. */
+__device__ static inline void stage_issue()
 {
     const int t = threadIdx.x, lane = (int)__lane_id();
     const int k = stg_cnt[t];
-    const uint64_t m1 = __ballot(k >= 1), m2 = __ballot(k >= 2);
+    const uint64_t m1 = __ballot(k >= 1);
     if (m1 == 0) return;
+    const uint64_t m2 = __ballot(k >= 2);
     const int n1 = __popcll(m1), tot = n1 + __popcll(m2);
     const uint64_t lt = lanemask_lt();
     int32_t *idx = stg_idx[t >> 6];
@@ -222,22 +222,12 @@ __device__ static inline void stage_issue(const DsmcEnv &E)
      * complete in order; the fence keeps the compiler from moving the reads up */
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    if (E.cellacc) {
-        const int word = lane & 3;
-        for (int g = 0; g < tot; g += 16) {
-            const int r = g + (lane >> 2);
-            if (r < tot) {
-                const int e = idx[r];
-                gadd(&E.cellacc[4 * (size_t)stg_cell[e] + word], stg_val[4 * e + word]);
-            }
-        }
-    } else {
-        for (int g = 0; g < tot; g += 64) {
-            const int r = g + lane;
-            if (r < tot) {
-                const int e = idx[r];
-                gadd(&E.vcount[stg_cell[e]], (unsigned long long)stg_val[4 * e + 3]);
-            }
+    const int word = lane & 3;
+    for (int g = 0; g < tot; g += 16) {
+        const int r = g + (lane >> 2);
+        if (r < tot) {
+            const int e = idx[r];
+            gadd((double *)stg_ptr[e] + word, stg_val[4 * e + word]);
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -407,7 +397,7 @@ lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batc
         TB.tq = lds_tabs;
         TB.hr = lds_tabs + ntq;
         TB.cs = lds_tabs + ntq + nhr;
-        stg_cnt[threadIdx.x] = 0;
+        stage_init();
         __syncthreads();
     }
     const unsigned int nq = Q.ctl[2 * p];
@@ -464,7 +454,7 @@ lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batc
                         else lean_refresh(N, L, E, TB);
                     }
                 }
-                stage_issue(E);
+                stage_issue();
             }
             uint64_t ended = __ballot(L.phase != LANE_FLIGHT && L.phase != LANE_IDLE);
             if (ended && (__popcll(ended | idle) >= refill_batch || __ballot(L.phase == LANE_FLIGHT) == 0)) break;
@@ -476,7 +466,7 @@ lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batc
             if (paused) L.phase = LANE_FLIGHT;
             rec_store_flight<1, 0>(L, rec[slot]);
         }
-        stage_issue(E);
+        stage_issue();
         wave_append(hop_end, slot, Q.svc, &Q.ctl[4]);
         wave_append(paused, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
         if (hop_end || paused) L.phase = LANE_IDLE;
@@ -634,7 +624,7 @@ __device__ __forceinline__ void lane_collide_lean(DsmcLane &L, const DsmcEnv *Ep
 /* Pools: up to 168 VGPRs (3 waves per SIMD), so that the lanes' per-run step
  * constants do not spill inside the flight loop (a reload waits behind the
  * wavefront's outstanding deposit atomics); single runs: 128 (4 waves). */
-template <bool POOL, int MODE, int ADV>
+template <bool POOL, int MODE, int ADV, bool STAGE>
 __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const DsmcEnv *Ed, const DsmcPoolDev &P,
                                                     DsmcRec *rec, const DsmcQueues &Q, int p, int svc_batch,
                                                     int64_t step_budget)
@@ -646,7 +636,9 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
     const double *gtq = POOL ? P.tq : E0.tq, *gcs = POOL ? P.cs : E0.cs, *ghr = POOL ? P.hr : E0.hr;
     for (int i = threadIdx.x; i < ntq + ncs + nhr; i += blockDim.x)
         lds_tabs[i] = i < ntq ? gtq[i] : (i < ntq + ncs ? gcs[i - ntq] : ghr[i - ntq - ncs]);
-    if (!POOL) dep_init();
+    static_assert(!STAGE || MODE == 1, "staged deposits are physical");
+    if constexpr (STAGE) stage_init();
+    else if constexpr (!POOL) dep_init();
     __syncthreads();
     const double *lds_hr = lds_tabs + ntq + ncs;
     DsmcCellTabs TB;
@@ -701,7 +693,9 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
 #pragma unroll
             for (int u = 0; u < 2; u++) {
                 if (L.phase == LANE_FLIGHT) {
-                    if constexpr (POOL) lane_step<DsmcOpsDev, 1, ADV>(L, S, TB);
+                    if constexpr (POOL && STAGE) lane_step<DsmcOpsStage, 1, ADV>(L, S, TB);
+                    else if constexpr (POOL) lane_step<DsmcOpsDev, 1, ADV>(L, S, TB);
+                    else if constexpr (STAGE) lane_step<DsmcOpsStage, 1, ADV>(L, E0, TB);
                     else lane_step<DsmcOpsLds, MODE, ADV>(L, E0, TB);
                     if (L.phase == LANE_FLIGHT && L.steps >= budget_end) L.phase = LANE_IDLE + 8;
                 }
@@ -712,6 +706,7 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
             if (MODE == 1 && ADV && __popcll(__ballot(L.phase == LANE_COLLIDE)) >= COLLIDE_BATCH &&
                 L.phase == LANE_COLLIDE)
                 lane_collide_lean(L, POOL ? P.envs + run : Ed);
+            if constexpr (STAGE) stage_issue();
             uint64_t ended = __ballot(L.phase != LANE_FLIGHT && L.phase != LANE_IDLE);
             uint64_t waiting = ended | __ballot(L.phase == LANE_IDLE);
             if ((ended && __popcll(waiting) >= svc_batch) || waiting == __ballot(true)) break;
@@ -725,10 +720,13 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
         if (svc) {
             DsmcLane T = L;
             if constexpr (MODE == 0) compat_service(T, Ed, cnt);
+            else if constexpr (POOL && STAGE) phys_service<DsmcOpsStage, ADV>(T, P.envs + run, cnt, TB);
             else if constexpr (POOL) phys_service<DsmcOpsDev, ADV>(T, P.envs + run, cnt, TB);
+            else if constexpr (STAGE) phys_service<DsmcOpsStage, ADV>(T, Ed, cnt, TB);
             else phys_service<DsmcOpsLds, ADV>(T, Ed, cnt, TB);
             L = T;
         }
+        if constexpr (STAGE) stage_issue();
         const bool fin = svc && L.phase == LANE_DONE;
         if (__ballot(fin)) counts_flush_wave(cnt, fin ? (POOL ? P.envs[run].counters : E0.counters) : nullptr, fin);
         if (fin) L.phase = LANE_IDLE;
@@ -738,26 +736,31 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
         bool paused = L.phase == LANE_IDLE + 8;
         if (paused) {
             L.phase = LANE_FLIGHT;
-            if (MODE == 0) {
+            if constexpr (MODE == 0) {
                 lane_flush<DsmcOpsLds>(L, E0);
                 rec_store_all<0>(L, rec[slot], 0);
-            } else rec_store_phys(L, rec[slot]);
+            } else {
+                rec_store_phys(L, rec[slot]);
+            }
         }
         wave_append(paused, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
         if (paused) L.phase = LANE_IDLE;
     }
-    if (!POOL) {
+    if constexpr (!POOL && !STAGE) {
         __syncthreads();
         dep_merge(E0);
     }
 }
 
-template <bool POOL, int ADV>
+/* STAGE: deposits to cell records staged and issued wavefront-wide (pools, and
+ * single runs with velocity moments); single runs without moments keep the
+ * LDS slot table for their count deposits. */
+template <bool POOL, int ADV, bool STAGE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(POOL ? 3 : 4)))
 phys_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcPoolDev P, DsmcRec *rec, DsmcQueues Q, int p, int svc_batch,
             int64_t step_budget)
 {
-    inline_service_loop<POOL, 1, ADV>(E0, Ed, P, rec, Q, p, svc_batch, step_budget);
+    inline_service_loop<POOL, 1, ADV, STAGE>(E0, Ed, P, rec, Q, p, svc_batch, step_budget);
 }
 
 /* Compat ballistic particles (the reference's run_ballistic) with the landing
@@ -769,7 +772,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 compat_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcRec *rec, DsmcQueues Q, int p, int svc_batch, int64_t step_budget)
 {
     DsmcPoolDev none = {};
-    inline_service_loop<false, 0, 0>(E0, Ed, none, rec, Q, p, svc_batch, step_budget);
+    inline_service_loop<false, 0, 0, false>(E0, Ed, none, rec, Q, p, svc_batch, step_budget);
 }
 
 /* ------------------------------------------------------------------ service */
@@ -809,8 +812,8 @@ __global__ void fill0_kernel(unsigned long long *p, size_t n)
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0ull;
 }
 
-/* Adds the lean flight kernel's per-cell records (sum vx, sum vy, sum vz, count)
- * to the run's moments and counts.  The counts are sums of integers below 2^53,
+/* Adds a physical run's per-cell records (sum vx, sum vy, sum vz, count) to its
+ * moments and counts.  The counts are sums of integers below 2^53,
  * exact in double precision. */
 __global__ void __launch_bounds__(256) fold_cellacc_kernel(const double *acc, double *vmom, unsigned long long *vcount,
                                                            int64_t cells)
@@ -930,8 +933,11 @@ extern "C" int dsmc_flight_occupancy(const DsmcEnv *E, int inline_svc, int block
     const size_t lds = sizeof(double) * (size_t)((E->ftd + 1) + (E->hd + 2) + 2 * (E->ffd + 1));
     const int mode = E->mode, adv = E->advanced;
     if (inline_svc && mode == 1) {
-        if (adv) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, phys_kernel<false, 1>, block, lds);
-        else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, phys_kernel<false, 0>, block, lds);
+        const bool stg = E->cellacc != nullptr;
+        if (adv && stg) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, phys_kernel<false, 1, true>, block, lds);
+        else if (adv) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, phys_kernel<false, 1, false>, block, lds);
+        else if (stg) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, phys_kernel<false, 0, true>, block, lds);
+        else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, phys_kernel<false, 0, false>, block, lds);
     } else if (inline_svc && !adv) {
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, compat_kernel, block, lds);
     } else if (mode == 0 && !adv) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flight_kernel<0, 0>, block, lds);
@@ -977,10 +983,10 @@ extern "C" hipError_t dsmc_launch_phys(const DsmcEnv *E, const DsmcEnv *Ed, cons
         DsmcEnv none;
         memset(&none, 0, sizeof none);
         if (P->adv)
-            hipLaunchKernelGGL((phys_kernel<true, 1>), dim3(grid), dim3(block), lds, st, none, (const DsmcEnv *)nullptr,
+            hipLaunchKernelGGL((phys_kernel<true, 1, true>), dim3(grid), dim3(block), lds, st, none, (const DsmcEnv *)nullptr,
                                *P, rec, Q, p, svc_batch, step_budget);
         else
-            hipLaunchKernelGGL((phys_kernel<true, 0>), dim3(grid), dim3(block), lds, st, none, (const DsmcEnv *)nullptr,
+            hipLaunchKernelGGL((phys_kernel<true, 0, true>), dim3(grid), dim3(block), lds, st, none, (const DsmcEnv *)nullptr,
                                *P, rec, Q, p, svc_batch, step_budget);
     } else if (E->mode == 0) {
         if (!Ed || E->advanced) return hipErrorInvalidValue;
@@ -992,12 +998,14 @@ extern "C" hipError_t dsmc_launch_phys(const DsmcEnv *E, const DsmcEnv *Ed, cons
         const size_t lds = sizeof(double) * (size_t)((E->ftd + 1) + 2 * (E->ffd + 1) + (E->hd + 2));
         DsmcPoolDev none;
         memset(&none, 0, sizeof none);
-        if (E->advanced)
-            hipLaunchKernelGGL((phys_kernel<false, 1>), dim3(grid), dim3(block), lds, st, *E, Ed, none, rec, Q, p,
-                               svc_batch, step_budget);
-        else
-            hipLaunchKernelGGL((phys_kernel<false, 0>), dim3(grid), dim3(block), lds, st, *E, Ed, none, rec, Q, p,
-                               svc_batch, step_budget);
+#define PHYS_SINGLE(A, S) hipLaunchKernelGGL((phys_kernel<false, A, S>), dim3(grid), dim3(block), lds, st, *E, Ed, \
+                                           none, rec, Q, p, svc_batch, step_budget)
+        const bool stg = E->cellacc != nullptr;
+        if (E->advanced && stg) PHYS_SINGLE(1, true);
+        else if (E->advanced) PHYS_SINGLE(1, false);
+        else if (stg) PHYS_SINGLE(0, true);
+        else PHYS_SINGLE(0, false);
+#undef PHYS_SINGLE
     }
     hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, Q.ctl, p, Q.ctl + 8);
     return hipGetLastError();

@@ -59,7 +59,7 @@ struct Pool {
     DevBuf geo_vert, geo_faces, geo_enorm, geo_fnorm, geo_fmap;   /* [max_geo] slots */
     int fmap_g = 0;                            /* face-map resolution of the class */
     DevBuf cdf, face_T;                        /* [max_runs][nfine] */
-    DevBuf vcount, vmom, horizon, counters;    /* [max_runs] grid slots */
+    DevBuf vcount, cellacc, horizon, counters; /* [max_runs] grid slots (cellacc: 32-byte cell records) */
     DevBuf end_state, steps_o, fate, hops;     /* [max_slots], optional */
 
     DsmcEnv *envs_h = nullptr;                 /* pinned host copies */

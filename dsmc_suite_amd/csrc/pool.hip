@@ -52,7 +52,7 @@ size_t pool_run_bytes(const PoolClass &c)
 {
     size_t cells = (size_t)c.ftd * c.ffd * c.hd;
     size_t nfine = (size_t)20 << (2 * c.levels);
-    return cells * (8 + 24) + (size_t)c.ftd * c.ffd * 8 + 64 + nfine * 16 + sizeof(DsmcEnv) +
+    return cells * (8 + 32) + (size_t)c.ftd * c.ffd * 8 + 64 + nfine * 16 + sizeof(DsmcEnv) +
            sizeof(DsmcStepEnv);
 }
 
@@ -226,7 +226,7 @@ int pool_create(Pool *P, const PoolClass &c, int max_runs, int64_t max_slots, in
     const size_t fmap_n = (size_t)6 * P->fmap_g * P->fmap_g * DSMC_FMAP_K;
     if ((rc = P->geo_fmap.alloc(POOL_MAX_GEO * fmap_n * 4))) return rc;
     if ((rc = P->cdf.alloc(mr * P->nfine * 8)) || (rc = P->face_T.alloc(mr * P->nfine * 8))) return rc;
-    if ((rc = P->vcount.alloc(mr * P->cells * 8)) || (rc = P->vmom.alloc(mr * P->cells * 24))) return rc;
+    if ((rc = P->vcount.alloc(mr * P->cells * 8)) || (rc = P->cellacc.alloc(mr * P->cells * 32))) return rc;
     if ((rc = P->horizon.alloc(mr * c.ftd * c.ffd * 8)) || (rc = P->counters.zeros(mr * 64, st))) return rc;
     if (per_particle) {
         if ((rc = P->end_state.alloc(ms * 48)) || (rc = P->steps_o.alloc(ms * 8)) || (rc = P->fate.alloc(ms * 4)) ||
@@ -260,7 +260,7 @@ int pool_create(Pool *P, const PoolClass &c, int max_runs, int64_t max_slots, in
         hipHostMalloc((void **)&P->ctl_h, POOL_RING * 16 * sizeof(unsigned int), 0) != hipSuccess)
         return DSMC_ERR_NOMEM;
     memset(P->cnt_h, 0, POOL_RING * mr * 64);
-    P->out_bytes = P->cells * 32 + (size_t)c.ftd * c.ffd * 8 + 64;
+    P->out_bytes = P->cells * 40 + (size_t)c.ftd * c.ffd * 8 + 64;
     if (hipHostMalloc(&P->out_h, P->out_bytes, 0) != hipSuccess) return DSMC_ERR_NOMEM;
     P->runs.assign(mr, PoolRun());
     P->free_slots.clear();
@@ -471,7 +471,9 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out, int hist, 
     E.density = nullptr;
     E.density_from_count = 1;
     E.vcount = (unsigned long long *)P->vcount.ptr + (size_t)g * cells;
-    E.vmom = pr.moments ? (double *)P->vmom.ptr + (size_t)g * cells * 3 : nullptr;
+    /* moments runs deposit into 32-byte cell records (vmom and vcount at collection) */
+    E.vmom = nullptr;
+    E.cellacc = pr.moments ? (double *)P->cellacc.ptr + (size_t)g * cells * 4 : nullptr;
     E.horizon = (double *)P->horizon.ptr + (size_t)g * P->cls.ftd * P->cls.ffd;
     E.counters = (unsigned long long *)P->counters.ptr + (size_t)g * 8;
     E.ttime = nullptr;
@@ -505,7 +507,7 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out, int hist, 
     S.h = E.h; S.GM = E.GM; S.c1 = E.c1; S.c2 = E.c2; S.R = E.R; S.max_height = E.max_height;
     S.sun_x = E.sun_x; S.sun_y = E.sun_y; S.max_steps = E.max_steps;
     S.ftd = E.ftd; S.ffd = E.ffd; S.hd = E.hd; S.density_from_count = 1;
-    S.density = nullptr; S.vmom = E.vmom; S.vcount = E.vcount;
+    S.density = nullptr; S.vmom = nullptr; S.vcount = E.vcount; S.cellacc = E.cellacc;
     S.hr_off = tab * hd2;
     S.nsig = E.nsig;
     S.env = hist >= 0 ? (const DsmcEnv *)P->envs_d.ptr + g : nullptr;
@@ -515,7 +517,7 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out, int hist, 
     HIP_OK(hipMemcpyAsync((DsmcStepEnv *)P->steps_d.ptr + g, &P->steps_h[g], sizeof(DsmcStepEnv),
                           hipMemcpyHostToDevice, st));
     HIP_OK(dsmc_launch_fill0(E.vcount, cells, st));
-    if (E.vmom) HIP_OK(dsmc_launch_fill0(E.vmom, cells * 3, st));
+    if (E.cellacc) HIP_OK(dsmc_launch_fill0(E.cellacc, cells * 4, st));
     HIP_OK(dsmc_launch_fill0(E.horizon, (size_t)P->cls.ftd * P->cls.ffd, st));
     HIP_OK(dsmc_launch_fill0(E.counters, 8, st));
     if (E.vhist) HIP_OK(dsmc_launch_fill0(E.vhist, cells * P->hist_bins, st));
@@ -598,19 +600,27 @@ int pool_collect(Pool *P, int g, dsmc_result *r)
     char *o = (char *)P->out_h;
     unsigned long long cnt[8];
     HIP_OK(hipMemcpyAsync(o, E.vcount, cells * 8, hipMemcpyDeviceToHost, cs));
-    if (E.vmom) HIP_OK(hipMemcpyAsync(o + cells * 8, E.vmom, cells * 24, hipMemcpyDeviceToHost, cs));
-    HIP_OK(hipMemcpyAsync(o + cells * 32, E.horizon, nh * 8, hipMemcpyDeviceToHost, cs));
-    HIP_OK(hipMemcpyAsync(o + cells * 32 + nh * 8, E.counters, sizeof cnt, hipMemcpyDeviceToHost, cs));
+    if (E.cellacc) HIP_OK(hipMemcpyAsync(o + cells * 8, E.cellacc, cells * 32, hipMemcpyDeviceToHost, cs));
+    HIP_OK(hipMemcpyAsync(o + cells * 40, E.horizon, nh * 8, hipMemcpyDeviceToHost, cs));
+    HIP_OK(hipMemcpyAsync(o + cells * 40 + nh * 8, E.counters, sizeof cnt, hipMemcpyDeviceToHost, cs));
     HIP_OK(hipStreamSynchronize(cs));
-    memcpy(cnt, o + cells * 32 + nh * 8, sizeof cnt);
-    const int64_t *vc = (const int64_t *)o;
+    memcpy(cnt, o + cells * 40 + nh * 8, sizeof cnt);
+    int64_t *vc = (int64_t *)o;
+    const double *acc = (const double *)(o + cells * 8);
+    /* a moments run's counts and velocity sums are in its cell records: exact
+     * integer counts (below 2^53) in the fourth word */
+    if (E.cellacc) for (size_t i = 0; i < cells; i++) vc[i] += (int64_t)acc[4 * i + 3];
     if (r->vcount) memcpy(r->vcount, vc, cells * 8);
     if (r->density) for (size_t i = 0; i < cells; i++) r->density[i] = (double)vc[i];
     if (r->vmom) {
-        if (E.vmom) memcpy(r->vmom, o + cells * 8, cells * 24);
-        else memset(r->vmom, 0, cells * 24);       /* a density-only run */
+        if (E.cellacc) {
+            for (size_t i = 0; i < cells; i++)
+                for (int k = 0; k < 3; k++) r->vmom[3 * i + k] = acc[4 * i + k];
+        } else {
+            memset(r->vmom, 0, cells * 24);       /* a density-only run */
+        }
     }
-    if (r->horizon) memcpy(r->horizon, o + cells * 32, nh * 8);
+    if (r->horizon) memcpy(r->horizon, o + cells * 40, nh * 8);
     const int64_t n = R.n;
     if (P->per_particle) {
         if (r->end_state) HIP_OK(hipMemcpy(r->end_state, E.end_state, n * 48, hipMemcpyDeviceToHost));
