@@ -324,11 +324,22 @@ static void collision_execute(V3 *v, V3 v2)
     v->z = vcm.z + 0.5 * del.z;
 }
 
+/* The dayside weight decay of a compat step.  The libm build evaluates
+ * exp(-h / lifetime) on every dayside step, as the reference does (so the CPU
+ * baseline does the reference's work); the device-math build multiplies by the
+ * factor the product tabulates once on the host with the same libm call (the
+ * device has no glibc).  Both give the same double. */
+#ifdef ORACLE_DEVMATH
+#define DAY_DECAY(c, h) ((c)->decay)
+#else
+#define DAY_DECAY(c, h) exp(-(h) / (c)->lifetime)
+#endif
+
 /* ------------------------------------------------------------------ run context */
 typedef struct {
     const dsmc_params *p;
     int ftd, ffd, hd, fit_deg;
-    double GM, R, h, decay, dt_rot, ve, max_height, unih, alpha2;
+    double GM, R, h, decay, lifetime, dt_rot, ve, max_height, unih, alpha2;
     const double *T;
     vtable *vt;
     double *density, *horizon, *vmom;
@@ -473,7 +484,7 @@ static void ballistic_particle(ctx_t *c, int64_t n, orng *rg)
             if (st == VS_ESCAPED) { ou = 1; break; }
             Ri = v3len(pp);
             step_cell(pp, Ri, ftd, ffd, &td, &fd);
-            if (fd < ffd / 2) W *= c->decay;
+            if (fd < ffd / 2) W *= DAY_DECAY(c, h);                 /* ballistic.c:203-204 */
             if (ii == 0) continue;
             int ht = (int)ceil((Ri - R) / c->unih) - 1;
             if (ht < 0) ht = 0;
@@ -600,7 +611,7 @@ static void advanced_particle(ctx_t *c, int64_t n, orng *rg)
             if (st == VS_ESCAPED) { ou = 1; break; }
             Ri = v3len(pp);
             step_cell(pp, Ri, ftd, ffd, &td, &fd);
-            if (fd < ffd / 2) W *= c->decay;
+            if (fd < ffd / 2) W *= DAY_DECAY(c, h);                 /* advanced.c:355-356 */
             if (ii == 0) continue;
             int ht = exp_ht_index(Ri, R, c->unih, c->alpha2);
             if (ht < 0) ht = 0;
@@ -953,6 +964,7 @@ static void ctx_common(ctx_t *c, const dsmc_params *p, dsmc_result *r)
     c->R = p->radius;
     c->h = p->dt;
     c->decay = exp(-p->dt / p->lifetime);
+    c->lifetime = p->lifetime;
     c->dt_rot = p->period / p->ffd;
     c->ve = p->v_esc;
     c->max_height = p->max_height;
