@@ -382,7 +382,10 @@ flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch, in
  * are staged in LDS and issued by the whole wavefront (stage_issue: four lanes
  * per cell record with moments), 158 against 182 ms of flight kernel per 1e7-
  * particle Ceres run with one lane's four atomics per visit. */
-#define LEAN_STEPS 2
+/* lean steps between two votes of the wavefront: on the Ceres 1e7 run 2 / 4 /
+ * 6 / 8 / 12 / 16 take 157 / 145 / 139 / 137 / 141 / 145 ms of flight kernel
+ * (a parked lane idles for the rest of its group; the vote costs every lane) */
+#define LEAN_STEPS 8
 #define LEAN_PARK_BATCH 4
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch, int64_t step_budget,
