@@ -1081,18 +1081,20 @@ DSMC_HD double verlet_push(double &sx, double &sy, double &sz, double &vx, doubl
                            double &oy, double &oz, double Ri, double GM, double h, double c1, double c2, double *ds)
 {
     const double g = dm_div_pos(GM, Ri * Ri * Ri);
-    const double ax = g * sx, ay = g * sy, az = g * sz;
-    const double nx = sx + vx * h - ax * c1 - ox * c2;
-    const double ny = sy + vy * h - ay * c1 - oy * c2;
-    const double nz = sz + vz * h - az * c1 - oz * c2;
+    /* the previous acceleration's term first, so the new acceleration can take
+     * its registers: the same products and sums, each rounded as before */
+    const double tx = ox * c2, ty = oy * c2, tz = oz * c2;
+    ox = g * sx; oy = g * sy; oz = g * sz;
+    const double nx = sx + vx * h - ox * c1 - tx;
+    const double ny = sy + vy * h - oy * c1 - ty;
+    const double nz = sz + vz * h - oz * c1 - tz;
     if (DS) {
         const double dx = nx - sx, dy = ny - sy, dz = nz - sz;
         *ds = sqrt(dx * dx + dy * dy + dz * dz);
     }
-    vx -= ax * h;
-    vy -= ay * h;
-    vz -= az * h;
-    ox = ax; oy = ay; oz = az;
+    vx -= ox * h;
+    vy -= oy * h;
+    vz -= oz * h;
     sx = nx; sy = ny; sz = nz;
     return dm_sqrt_pos(nx * nx + ny * ny + nz * nz);
 }
@@ -1169,6 +1171,8 @@ DSMC_HD void lane_step(DsmcLane &L, const EnvT &E, const DsmcCellTabs &TB)
 struct LeanLane {
     CellPre B;
     int64_t stop;
+    int32_t sun_steps;  /* sunlit lean steps since the refresh, not yet in L.n_day */
+    int32_t sun_lim;    /* L.n_photo - L.n_day - 1 at the refresh, at most INT32_MAX */
     bool easy;      /* the next steps may take lane_fast_phys's shortcut */
     bool pend;      /* a pushed step waits for lane_tail_phys */
 };
@@ -1179,6 +1183,16 @@ DSMC_HD void lean_refresh(LeanLane &N, const DsmcLane &L, const EnvT &E, const D
 {
     N.B = cell_pre(L.td, L.fd, L.ht, TB.tq, TB.cs, TB.hr);
     N.easy = L.ht >= E.hd || L.acc_cell == (L.ht * E.ftd + L.td) * E.ffd + L.fd;
+    const int64_t lim = L.n_photo - L.n_day - 1;
+    N.sun_lim = lim < (int64_t)INT32_MAX ? (int32_t)lim : INT32_MAX;
+    N.sun_steps = 0;
+}
+
+/* Adds the lean steps' sunlit count to the lane (before lane_tail_phys reads it). */
+DSMC_HD void lean_sync(LeanLane &N, DsmcLane &L)
+{
+    L.n_day += N.sun_steps;
+    N.sun_steps = 0;
 }
 
 /* One physical step (ballistic, density and moments by visit) whose outcome is,
@@ -1186,12 +1200,12 @@ DSMC_HD void lean_refresh(LeanLane &N, const DsmcLane &L, const EnvT &E, const D
  * is firmly inside the current cell (the first tests of td_update_pre,
  * fd_update_pre and table_walk_pre, on the entries the lane holds), neither
  * landed nor escaped, not photolysed on this step, short of the stop step, and
- * the open visit is this cell's - the sunlit count and the visit's sums, exactly
- * as lane_tail_phys would apply them, and true.  Otherwise only the push is
- * applied and it returns false: lane_tail_phys(L, ..., N.B, L.Ri, 0) completes
- * the step. */
+ * the open visit is this cell's - the sunlit count (held in N.sun_steps until
+ * lean_sync) and the visit's sums, exactly as lane_tail_phys would apply them,
+ * and true.  Otherwise only the push is applied and it returns false: lean_sync,
+ * then lane_tail_phys(L, ..., N.B, L.Ri, 0) completes the step. */
 template <class EnvT>
-DSMC_HD bool lane_fast_phys(DsmcLane &L, const EnvT &E, const LeanLane &N)
+DSMC_HD bool lane_fast_phys(DsmcLane &L, const EnvT &E, LeanLane &N)
 {
     L.steps++;
     const double Rn = verlet_push<false>(L.sx, L.sy, L.sz, L.vx, L.vy, L.vz, L.ox, L.oy, L.oz, L.Ri, E.GM, E.h,
@@ -1204,10 +1218,12 @@ DSMC_HD bool lane_fast_phys(DsmcLane &L, const EnvT &E, const LeanLane &N)
     const CellPre &B = N.B;
     const bool stay = (z - B.tlo * Rn > et) & (B.thi * Rn - z > et) & (x * B.c0 - y * B.s0 > ef) &
                       (x * B.c1 - y * B.s1 < -ef) & (Rn >= B.rlo) & (Rn < B.rhi);
+    /* photolysed on this step: the sunlit count L.n_day + N.sun_steps + 1 reaches
+     * n_photo (with sun_lim clamped, a lane parks before sun_steps overflows) */
     const bool ok = stay & (Rn > E.R) & (Rn < E.max_height) & (L.steps < N.stop) & N.easy &
-                    !(sun & (L.n_day + 1 >= L.n_photo));
+                    !(sun & (N.sun_steps >= N.sun_lim));
     if (ok) {
-        L.n_day += sun ? 1 : 0;
+        N.sun_steps += sun ? 1 : 0;
         if (L.ht < E.hd) {
             L.acx += L.vx; L.acy += L.vy; L.acz += L.vz;
             L.acc_n++;

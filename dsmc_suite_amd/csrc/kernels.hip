@@ -444,12 +444,17 @@ lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batc
             continue;
         }
         for (;;) {
+            /* `act` stays a lane mask across the unrolled steps */
+            const bool fl = L.phase == LANE_FLIGHT;
+            bool act = fl && !N.pend;
 #pragma unroll
             for (int u = 0; u < LEAN_STEPS; u++)
-                if (L.phase == LANE_FLIGHT && !N.pend) N.pend = !lane_fast_phys(L, E, N);
+                if (act) act = lane_fast_phys(L, E, N);
+            N.pend = fl && !act;
             const uint64_t parked = __ballot(N.pend);
-            if (parked && (__popcll(parked) >= pend_batch || parked == __ballot(L.phase == LANE_FLIGHT))) {
+            if (parked && (__popcll(parked) >= pend_batch || parked == __ballot(fl))) {
                 if (N.pend) {
+                    lean_sync(N, L);
                     lane_tail_phys<DsmcOpsStage, 0>(L, E, TB, N.B, L.Ri, 0.0);
                     N.pend = false;
                     if (L.phase == LANE_FLIGHT) {

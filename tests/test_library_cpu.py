@@ -247,19 +247,23 @@ def test_lane_state_machine_matches_oracle_physical(lib, emu):
     assert np.allclose(got.density, want.density, rtol=1e-12, atol=0)
 
 
-@pytest.mark.parametrize("body", [0, 3])
+@pytest.mark.parametrize("body,lifetime", [(0, None), (3, None), (0, 1e30)])
 @pytest.mark.parametrize("budget", [23, 4096])
-def test_lean_flight_path_matches_oracle(lib, emu, body, budget):
+def test_lean_flight_path_matches_oracle(lib, emu, body, lifetime, budget):
     """The event-batched physical flight kernel's lane sequence (lean steps
-    while nothing but the step count and open visit change, the full step logic
-    when one needs it, the cell entries reloaded after it; records and the
-    service kernel between launches): bit-identical to the oracle for a
-    long-hop body (Ceres) and a short-hop one (Mercury), with velocity moments."""
+    while nothing but the step count, the sunlit count and the open visit
+    change, the full step logic when one needs it, the cell entries reloaded
+    after it; records and the service kernel between launches): bit-identical
+    to the oracle for a long-hop body (Ceres) and a short-hop one (Mercury),
+    with velocity moments; and with a photolysis lifetime so long that the
+    lean steps' sunlit-count limit is clamped to INT32_MAX."""
     from dsmc_suite_amd import workloads as W
     dev = oracle(devmath=True)
     b = W.BODIES[body]
     surf = W.icosphere(lib, 3, b, 2.0)
     p = W.body_params(lib, b, 150, 2.0, 91, hd=40)
+    if lifetime:
+        p.lifetime = lifetime
     want = run_oracle(dev, p, surf, nthreads=8, moments=True)
     got = Buffers(p, 5, moments=True)
     emu.emu_ballistic_path.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int64]
