@@ -1162,33 +1162,62 @@ DSMC_HD void lane_step(DsmcLane &L, const EnvT &E, const DsmcCellTabs &TB)
 }
 
 /* ------------------------------------------------------------------ lean physical step */
-/* What a lane of the event-batched flight loops holds besides DsmcLane: the
- * current cell's table entries (in registers, so a step reads no table), the
- * step at which the lane must stop (the step cap or the launch's budget,
- * whichever comes first) and whether a step may accumulate into the open visit
- * without the cell logic: the cell is above the grid (no deposit), or it is
- * the open visit's cell. */
+/* What a lane of the event-batched flight loops holds besides DsmcLane: its
+ * cell's lean bounds (in registers, so a step reads no table), the step at which
+ * the lane must stop (the step cap or the launch's budget, whichever comes
+ * first), the sunlit steps taken since the last refresh and whether a step may
+ * accumulate into the open visit without the cell logic: the cell is above the
+ * grid (no deposit), or it is the open visit's cell.
+ *
+ * Lean bounds: each of the cell tests of a full step (td_update_pre,
+ * fd_update_pre, table_walk_pre: inside by a margin of 1e-12 relative, landed,
+ * escaped) reduced to one product and one comparison, by bounds moved inward:
+ * latitude thresholds by LEAN_TQ_SHIFT (z > tl Rn implies z - tlo Rn > 1e-12 Rn
+ * with every rounding counted: the shift is 4e-12 against a margin of 1e-12
+ * and roundings of ~1e-16); sector boundary directions rotated inward by an
+ * angle of sine LEAN_CS_SHIFT (x c0' - y s0' > 0 gives rho sin(psi - beta0) >
+ * ~4e-12 rho > 1e-12 (|x| + |y|)); the radial range intersected with (R,
+ * max_height), exactly (Rn > R is Rn >= the next double above R).  A step that
+ * passes them passes the full step's tests: only the converse can fail, and a
+ * lane that fails them parks and runs the full step.  The full step's first
+ * tests (the _pre functions) take the same moved bounds: inside them a point is
+ * inside the cell, and otherwise those functions fall back to the exact walks
+ * on the tables, so the cell they return is the one the original bounds give
+ * (lane_tail_phys tests landing and escape before the radial walk). */
+#define LEAN_TQ_SHIFT 4e-12
+#define LEAN_CS_SHIFT 4e-12
+
 struct LeanLane {
-    CellPre B;
+    CellPre B;      /* the lean bounds, in CellPre's layout */
     int64_t stop;
     int32_t sun_steps;  /* sunlit lean steps since the refresh, not yet in L.n_day */
     int32_t sun_lim;    /* L.n_photo - L.n_day - 1 at the refresh, at most INT32_MAX */
     bool easy;      /* the next steps may take lane_fast_phys's shortcut */
-    bool pend;      /* a pushed step waits for lane_tail_phys */
+    bool pend;      /* a pushed step waits for lean_tail */
 };
 
-/* Refreshes the cell entries and the shortcut flag after the cell logic ran. */
+/* Refreshes the lean bounds, the shortcut flag and the sunlit limit after the
+ * cell logic ran. */
 template <class EnvT>
 DSMC_HD void lean_refresh(LeanLane &N, const DsmcLane &L, const EnvT &E, const DsmcCellTabs &TB)
 {
-    N.B = cell_pre(L.td, L.fd, L.ht, TB.tq, TB.cs, TB.hr);
+    const CellPre P = cell_pre(L.td, L.fd, L.ht, TB.tq, TB.cs, TB.hr);
+    N.B.tlo = P.tlo + LEAN_TQ_SHIFT;
+    N.B.thi = P.thi - LEAN_TQ_SHIFT;
+    N.B.c0 = P.c0 - P.s0 * LEAN_CS_SHIFT;
+    N.B.s0 = P.s0 + P.c0 * LEAN_CS_SHIFT;
+    N.B.c1 = P.c1 + P.s1 * LEAN_CS_SHIFT;
+    N.B.s1 = P.s1 - P.c1 * LEAN_CS_SHIFT;
+    const double r_up = nextafter(E.R, INFINITY);          /* Rn > R  <=>  Rn >= r_up */
+    N.B.rlo = P.rlo > r_up ? P.rlo : r_up;
+    N.B.rhi = P.rhi < E.max_height ? P.rhi : E.max_height;
     N.easy = L.ht >= E.hd || L.acc_cell == (L.ht * E.ftd + L.td) * E.ffd + L.fd;
     const int64_t lim = L.n_photo - L.n_day - 1;
     N.sun_lim = lim < (int64_t)INT32_MAX ? (int32_t)lim : INT32_MAX;
     N.sun_steps = 0;
 }
 
-/* Adds the lean steps' sunlit count to the lane (before lane_tail_phys reads it). */
+/* Adds the lean steps' sunlit count to the lane (before the cell logic reads it). */
 DSMC_HD void lean_sync(LeanLane &N, DsmcLane &L)
 {
     L.n_day += N.sun_steps;
@@ -1197,13 +1226,12 @@ DSMC_HD void lean_sync(LeanLane &N, DsmcLane &L)
 
 /* One physical step (ballistic, density and moments by visit) whose outcome is,
  * in most steps, decided by a few comparisons: the push, then - when the point
- * is firmly inside the current cell (the first tests of td_update_pre,
- * fd_update_pre and table_walk_pre, on the entries the lane holds), neither
- * landed nor escaped, not photolysed on this step, short of the stop step, and
- * the open visit is this cell's - the sunlit count (held in N.sun_steps until
+ * passes the lean bounds (firmly inside the current cell, neither landed nor
+ * escaped), is not photolysed on this step, is short of the stop step, and the
+ * open visit is this cell's - the sunlit count (held in N.sun_steps until
  * lean_sync) and the visit's sums, exactly as lane_tail_phys would apply them,
- * and true.  Otherwise only the push is applied and it returns false: lean_sync,
- * then lane_tail_phys(L, ..., N.B, L.Ri, 0) completes the step. */
+ * and true.  Otherwise only the push is applied and it returns false: lean_tail
+ * completes the step. */
 template <class EnvT>
 DSMC_HD bool lane_fast_phys(DsmcLane &L, const EnvT &E, LeanLane &N)
 {
@@ -1213,15 +1241,12 @@ DSMC_HD bool lane_fast_phys(DsmcLane &L, const EnvT &E, LeanLane &N)
     L.Ri = Rn;
     const double x = L.sx, y = L.sy, z = L.sz;
     const bool sun = x * E.sun_x + y * E.sun_y > 0.0;
-    const double et = 1e-12 * Rn;
-    const double ef = DSMC_FD_MARGIN * (fabs(x) + fabs(y));
     const CellPre &B = N.B;
-    const bool stay = (z - B.tlo * Rn > et) & (B.thi * Rn - z > et) & (x * B.c0 - y * B.s0 > ef) &
-                      (x * B.c1 - y * B.s1 < -ef) & (Rn >= B.rlo) & (Rn < B.rhi);
+    const bool stay = (z > B.tlo * Rn) & (z < B.thi * Rn) & (x * B.c0 - y * B.s0 > 0.0) &
+                      (x * B.c1 - y * B.s1 < 0.0) & (Rn >= B.rlo) & (Rn < B.rhi);
     /* photolysed on this step: the sunlit count L.n_day + N.sun_steps + 1 reaches
      * n_photo (with sun_lim clamped, a lane parks before sun_steps overflows) */
-    const bool ok = stay & (Rn > E.R) & (Rn < E.max_height) & (L.steps < N.stop) & N.easy &
-                    !(sun & (N.sun_steps >= N.sun_lim));
+    const bool ok = stay & (L.steps < N.stop) & N.easy & !(sun & (N.sun_steps >= N.sun_lim));
     if (ok) {
         N.sun_steps += sun ? 1 : 0;
         if (L.ht < E.hd) {
@@ -1230,6 +1255,15 @@ DSMC_HD bool lane_fast_phys(DsmcLane &L, const EnvT &E, LeanLane &N)
         }
     }
     return ok;
+}
+
+/* The rest of a parked lane's step: the full step logic (lane_tail_phys),
+ * its first cell tests on the lean bounds. */
+template <class Ops, class EnvT>
+DSMC_HD void lean_tail(DsmcLane &L, const EnvT &E, const DsmcCellTabs &TB, LeanLane &N)
+{
+    lean_sync(N, L);
+    lane_tail_phys<Ops, 0>(L, E, TB, N.B, L.Ri, 0.0);
 }
 
 /* The rest of a compat step (ballistic.c:195-236 / advanced.c:347-402). */

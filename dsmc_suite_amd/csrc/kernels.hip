@@ -454,8 +454,7 @@ lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batc
             const uint64_t parked = __ballot(N.pend);
             if (parked && (__popcll(parked) >= pend_batch || parked == __ballot(fl))) {
                 if (N.pend) {
-                    lean_sync(N, L);
-                    lane_tail_phys<DsmcOpsStage, 0>(L, E, TB, N.B, L.Ri, 0.0);
+                    lean_tail<DsmcOpsStage>(L, E, TB, N);
                     N.pend = false;
                     if (L.phase == LANE_FLIGHT) {
                         if (L.steps >= budget_end) L.phase = LANE_IDLE + 8;

@@ -96,8 +96,7 @@ static void run_one_lean(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t b
         bool paused = false;
         while (F.phase == LANE_FLIGHT) {
             if (lane_fast_phys(F, E, N)) continue;
-            lean_sync(N, F);
-            lane_tail_phys<DsmcOpsHost, 0>(F, E, TB, N.B, F.Ri, 0.0);
+            lean_tail<DsmcOpsHost>(F, E, TB, N);
             if (F.phase != LANE_FLIGHT) break;
             if (F.steps >= end) { paused = true; break; }
             lean_refresh(N, F, E, TB);
