@@ -1240,10 +1240,13 @@ DSMC_HD bool lane_fast_phys(DsmcLane &L, const EnvT &E, LeanLane &N)
                                          E.c1, E.c2, nullptr);
     L.Ri = Rn;
     const double x = L.sx, y = L.sy, z = L.sz;
-    const bool sun = x * E.sun_x + y * E.sun_y > 0.0;
+    /* fl(a + b) > 0 exactly when a > -b, and fl(a - b) > 0 exactly when a > b
+     * (a rounded sum or difference keeps the sign of the exact one): the same
+     * decisions as lane_tail_phys's sun test and the sector tests' differences */
+    const bool sun = x * E.sun_x > -(y * E.sun_y);
     const CellPre &B = N.B;
-    const bool stay = (z > B.tlo * Rn) & (z < B.thi * Rn) & (x * B.c0 - y * B.s0 > 0.0) &
-                      (x * B.c1 - y * B.s1 < 0.0) & (Rn >= B.rlo) & (Rn < B.rhi);
+    const bool stay = (z > B.tlo * Rn) & (z < B.thi * Rn) & (x * B.c0 > y * B.s0) & (x * B.c1 < y * B.s1) &
+                      (Rn >= B.rlo) & (Rn < B.rhi);
     /* photolysed on this step: the sunlit count L.n_day + N.sun_steps + 1 reaches
      * n_photo (with sun_lim clamped, a lane parks before sun_steps overflows) */
     const bool ok = stay & (L.steps < N.stop) & N.easy & !(sun & (N.sun_steps >= N.sun_lim));

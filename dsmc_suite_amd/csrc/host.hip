@@ -408,11 +408,11 @@ dsmc_launch_shape default_shape(const DsmcEnv &E, int short_hops)
         s.service_inline = 1;
     } else {
         /* long hops (Ceres), 4e6 particles and more: the lean flight kernel.  1e7
-         * particles: 5 blocks per CU and a 4096-step budget in the first two
-         * rounds (1024 after them) 1.31e11 particle-steps/s (4 blocks, 1024 steps
-         * throughout: 1.13e11; the service inline: 203 ms per run, a tie) */
+         * particles: 5 blocks per CU, a 4096-step budget in the first two rounds
+         * (1024 after them), refills of 12 idle lanes: 124 ms of flight kernel per
+         * run (refills of 4 / 8 / 16 / 32 lanes: 165 / 125 / 126 / 137 ms) */
         s.blocks_per_cu = 5;
-        s.relaunch_batch = 16;
+        s.relaunch_batch = 12;
         s.step_budget = 4096;
         s.service_inline = 0;
         s.late_step_budget = 1024;
