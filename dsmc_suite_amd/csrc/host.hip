@@ -400,8 +400,9 @@ dsmc_launch_shape default_shape(const DsmcEnv &E, int short_hops)
         s.service_inline = 1;
     } else if (E.nparticles < 4000000) {
         /* long hops (Ceres), runs below 4e6 particles: the landing service inline
-         * in batches of 24 lanes, one launch (3e6 particles 82 ms against 87 ms
-         * with the split kernels, 1e6 44 against 47, 3e4 21 against 24) */
+         * in batches of 24 lanes, one launch (measured with the full-step loop:
+         * 3e6 particles 82 ms against 87 ms with the split kernels, 1e6 44 against
+         * 47, 3e4 21 against 24; with lean steps 1e6 takes 32 ms) */
         s.blocks_per_cu = 4;
         s.relaunch_batch = 24;
         s.step_budget = INT_MAX;
