@@ -1196,6 +1196,22 @@ struct LeanLane {
     bool pend;      /* a pushed step waits for lean_tail */
 };
 
+/* What lane_fast_phys reads of a run's constants (a pool lane holds these in
+ * registers and reads its run's full step constants only for the cell logic). */
+struct LeanEnv {
+    double GM, h, c1, c2, sun_x, sun_y;
+    int hd;
+};
+
+template <class EnvT>
+DSMC_HD LeanEnv lean_env(const EnvT &E)
+{
+    LeanEnv F;
+    F.GM = E.GM; F.h = E.h; F.c1 = E.c1; F.c2 = E.c2; F.sun_x = E.sun_x; F.sun_y = E.sun_y;
+    F.hd = E.hd;
+    return F;
+}
+
 /* Refreshes the lean bounds, the shortcut flag and the sunlit limit after the
  * cell logic ran. */
 template <class EnvT>

@@ -391,6 +391,9 @@ flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch, in
 #ifndef LEAN_INLINE
 #define LEAN_INLINE 1
 #endif
+#ifndef LEAN_POOL
+#define LEAN_POOL 1
+#endif
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch, int64_t step_budget,
                    int pend_batch)
@@ -662,16 +665,19 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
     unsigned int *head = &Q.ctl[2 * p + 1];
     const int32_t *flyq = Q.fly[p];
     const uint32_t lane = __lane_id();
-    /* single physical ballistic runs take lean steps (lane_fast_phys) and run the
+    /* physical ballistic particles take lean steps (lane_fast_phys) and run the
      * full step logic in batches of parked lanes, as lean_flight_kernel does
-     * (1e6-particle Ceres run +33%, the 64 runs of the batch one by one +22%);
-     * pools keep the full step: their lanes hold their run's step constants too,
-     * and there the lean loop was 22% slower on the 64-run batch */
-    constexpr bool LEAN = !POOL && MODE == 1 && ADV == 0 && LEAN_INLINE;
+     * (1e6-particle Ceres run +33%, the 64 runs of the batch one by one +22%,
+     * the 64-run pool +10%, +16% with moments).  A pool lane holds only the
+     * lean steps' run constants (LeanEnv) and reads its run's full step
+     * constants again for the cell logic: with all of them in registers the
+     * lean steps spilled and the pool ran 22% slower */
+    constexpr bool LEAN = (!POOL || LEAN_POOL) && MODE == 1 && ADV == 0 && LEAN_INLINE;
     DsmcLane L;
     L.phase = LANE_IDLE;
     DsmcStepEnv S = {};
     LeanLane N;
+    LeanEnv FE = {};          /* pools: the lean steps' run constants */
     N.pend = false;
     N.easy = false;
     N.stop = 0;
@@ -691,7 +697,10 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
                 unsigned int qi = base + __popcll(idle & lanemask_lt());
                 if (qi < nq) {
                     slot = flyq[qi];
-                    if (POOL) {
+                    if constexpr (POOL && LEAN) {
+                        run = rec[slot].run;
+                        TB.hr = lds_hr + P.steps[run].hr_off;
+                    } else if (POOL) {
                         run = rec[slot].run;
                         S = P.steps[run];
                         TB.hr = lds_hr + S.hr_off;
@@ -699,13 +708,18 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
                     if (MODE == 0) rec_load_all<0>(L, rec[slot]);
                     else rec_load_phys(L, rec[slot]);
                     budget_end = step_budget > 0 ? L.steps + step_budget : INT64_MAX;
-                    if constexpr (LEAN) {
-                        const int64_t mx = POOL ? S.max_steps : E0.max_steps;
-                        const int64_t cap = mx > 0 ? mx : INT64_MAX;
+                    if constexpr (LEAN && POOL) {
+                        const DsmcStepEnv Sr = P.steps[run];
+                        const int64_t cap = Sr.max_steps > 0 ? Sr.max_steps : INT64_MAX;
                         N.stop = budget_end < cap ? budget_end : cap;
                         N.pend = false;
-                        if constexpr (POOL) lean_refresh(N, L, S, TB);
-                        else lean_refresh(N, L, E0, TB);
+                        FE = lean_env(Sr);
+                        lean_refresh(N, L, Sr, TB);
+                    } else if constexpr (LEAN) {
+                        const int64_t cap = E0.max_steps > 0 ? E0.max_steps : INT64_MAX;
+                        N.stop = budget_end < cap ? budget_end : cap;
+                        N.pend = false;
+                        lean_refresh(N, L, E0, TB);
                     }
                 }
             }
@@ -715,7 +729,7 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
             if (exhausted) break;
             continue;
         }
-        if (POOL) { S.ftd = P.ftd; S.ffd = P.ffd; S.hd = P.hd; }
+        if (POOL && !LEAN) { S.ftd = P.ftd; S.ffd = P.ffd; S.hd = P.hd; }
         for (;;) {
           if constexpr (LEAN) {
             const bool fl = L.phase == LANE_FLIGHT;
@@ -723,7 +737,7 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
 #pragma unroll
             for (int u = 0; u < LEAN_STEPS; u++) {
                 if (act) {
-                    if constexpr (POOL) act = lane_fast_phys(L, S, N);
+                    if constexpr (POOL) act = lane_fast_phys(L, FE, N);
                     else act = lane_fast_phys(L, E0, N);
                 }
             }
@@ -731,15 +745,25 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
             const uint64_t parked = __ballot(N.pend);
             if (parked && (__popcll(parked) >= LEAN_PARK_BATCH || parked == __ballot(fl))) {
                 if (N.pend) {
-                    if constexpr (POOL && STAGE) lean_tail<DsmcOpsStage>(L, S, TB, N);
-                    else if constexpr (POOL) lean_tail<DsmcOpsDev>(L, S, TB, N);
-                    else if constexpr (STAGE) lean_tail<DsmcOpsStage>(L, E0, TB, N);
-                    else lean_tail<DsmcOpsLds>(L, E0, TB, N);
-                    N.pend = false;
-                    if (L.phase == LANE_FLIGHT) {
-                        if (L.steps >= budget_end) L.phase = LANE_IDLE + 8;
-                        else if constexpr (POOL) lean_refresh(N, L, S, TB);
-                        else lean_refresh(N, L, E0, TB);
+                    if constexpr (POOL) {
+                        /* the run's step constants, read again for the cell logic */
+                        DsmcStepEnv Sr = P.steps[run];
+                        Sr.ftd = P.ftd; Sr.ffd = P.ffd; Sr.hd = P.hd;
+                        if constexpr (STAGE) lean_tail<DsmcOpsStage>(L, Sr, TB, N);
+                        else lean_tail<DsmcOpsDev>(L, Sr, TB, N);
+                        N.pend = false;
+                        if (L.phase == LANE_FLIGHT) {
+                            if (L.steps >= budget_end) L.phase = LANE_IDLE + 8;
+                            else lean_refresh(N, L, Sr, TB);
+                        }
+                    } else {
+                        if constexpr (STAGE) lean_tail<DsmcOpsStage>(L, E0, TB, N);
+                        else lean_tail<DsmcOpsLds>(L, E0, TB, N);
+                        N.pend = false;
+                        if (L.phase == LANE_FLIGHT) {
+                            if (L.steps >= budget_end) L.phase = LANE_IDLE + 8;
+                            else lean_refresh(N, L, E0, TB);
+                        }
                     }
                 }
                 if constexpr (STAGE) stage_issue();
@@ -786,8 +810,13 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
             if constexpr (LEAN) {
                 if (L.phase == LANE_FLIGHT) {       /* relaunched: a new hop from its new cell */
                     N.pend = false;
-                    if constexpr (POOL) lean_refresh(N, L, S, TB);
-                    else lean_refresh(N, L, E0, TB);
+                    if constexpr (POOL) {
+                        DsmcStepEnv Sr = P.steps[run];
+                        Sr.ftd = P.ftd; Sr.ffd = P.ffd; Sr.hd = P.hd;
+                        lean_refresh(N, L, Sr, TB);
+                    } else {
+                        lean_refresh(N, L, E0, TB);
+                    }
                 }
             }
         }
