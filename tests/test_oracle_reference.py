@@ -18,7 +18,10 @@ from tests.helpers import (GOLDEN, ORNG_RAND_R, ORNG_SEQUENTIAL, c_fmt, compat_p
 
 @pytest.fixture(scope="module")
 def lib():
-    return oracle(devmath=False)
+    lib = oracle(devmath=False)
+    lib.cpu_ref_verlet_steps.argtypes = [C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double,
+                                         C.c_double, C.c_int, C.POINTER(C.c_double)]
+    return lib
 
 
 @pytest.mark.parametrize("tname", ["T_thermal", "T_polar"])
@@ -95,8 +98,6 @@ def test_verlet_trace_bitexact(lib):
     tr = fx["trace"]
     p = compat_params(36, 72, 1, 0)
     GM = p.G * p.M
-    lib.cpu_ref_verlet_steps.argtypes = [C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double,
-                                         C.c_double, C.c_int, C.POINTER(C.c_double)]
     checked = 0
     for i in range(2, len(tr)):
         if tr[i - 2, 6] != 0 or tr[i - 1, 6] != 0:
