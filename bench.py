@@ -582,6 +582,8 @@ def main():
     ap.add_argument("--service-inline", type=int, default=0)
     ap.add_argument("--late-step-budget", type=int, default=0, help="explicit shape: budget after two rounds")
     ap.add_argument("--park-batch", type=int, default=0, help="explicit shape: parked lanes per cell-logic pass")
+    ap.add_argument("--late-inline", type=int, default=0,
+                    help="explicit shape: rounds after the first two in one launch with the service inline")
     ap.add_argument("--cpu-sample", type=int, default=1500000,
                     help="particles of the all-core CPU baseline sample (~15 s on 16 cores)")
     ap.add_argument("--cpu-single-sample", type=int, default=60000,
@@ -621,10 +623,10 @@ def main():
     surf = icosphere(lib, 3)
     shape = None
     if (args.block or args.blocks_per_cu or args.relaunch_batch or args.step_budget or args.service_inline or
-            args.late_step_budget or args.park_batch):
+            args.late_step_budget or args.park_batch or args.late_inline):
         shape = capi.LaunchShape(args.block or 256, args.blocks_per_cu or 4, args.relaunch_batch or 16,
                                  args.step_budget or 1024, args.service_inline, args.late_step_budget,
-                                 args.park_batch)
+                                 args.park_batch, args.late_inline)
     progress(f"plain run at {args.particles:.0e} particles")
     main_run = plain_run(lib, surf, args.particles, args, shape)
     progress(f"plain: {main_run['value']:.3e} particle-steps/s")

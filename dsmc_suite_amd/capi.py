@@ -83,7 +83,7 @@ class Phase2Input(C.Structure):
 class LaunchShape(C.Structure):
     _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("relaunch_batch", C.c_int),
                 ("step_budget", C.c_int), ("service_inline", C.c_int), ("late_step_budget", C.c_int),
-                ("park_batch", C.c_int)]
+                ("park_batch", C.c_int), ("late_inline", C.c_int)]
 
 
 class ServerStats(C.Structure):

@@ -266,6 +266,11 @@ RunDev::~RunDev()
     if (host_ctl && ctl_pinned) (void)hipHostFree(host_ctl);
 }
 
+/* Lanes whose hop ended that a wavefront of the late inline launch gathers
+ * before running the landing service (the default of long-hop runs below 4e6
+ * particles, which take that kernel from their start). */
+#define LATE_INLINE_SVC_BATCH 24
+
 /* Runs a staged run to completion on stream st: the start kernel, then rounds of
  * flight + service until no particle is in flight.  Completion is checked on the
  * host every `chunk` rounds (doubling up to 128), so a run costs few syncs. */
@@ -317,6 +322,17 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
      * 8192-20000 in the later rounds) */
     const int64_t sh_budget = sh.step_budget;
     const int64_t late_budget = sh.late_step_budget > 0 ? sh.late_step_budget : sh_budget;
+    /* late_inline: the rounds after the first two run as launches of the kernel
+     * with the landing service inline and no step budget, so the particles still
+     * flying (most of them multi-hop, each hop of which would otherwise wait for
+     * a round's end) finish in the first such launch; it holds fewer blocks per
+     * CU than the lean flight kernel */
+    const bool late_inline = !inline_svc && sh.late_inline && rd->E.mode == DSMC_MODE_PHYSICAL && !rd->E.advanced;
+    int late_grid = flight_grid;
+    if (late_inline) {
+        const int occ = dsmc_flight_occupancy(&rd->E, 1, sh.block);
+        if (occ > 0 && (int64_t)cus * occ < (int64_t)late_grid) late_grid = cus * occ;
+    }
     /* per-launch timing events, destroyed on every return path */
     struct Events {
         std::vector<hipEvent_t> v;
@@ -337,6 +353,10 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
             if (inline_svc) {
                 HIP_OK(dsmc_launch_phys(&rd->E, (const DsmcEnv *)rd->env_d.ptr, nullptr, rec, Q, p, flight_grid,
                                         sh.block, sh.relaunch_batch, sh_budget, st));
+                if (timed) HIP_OK(hipEventRecord(ev.back(), st));
+            } else if (late_inline && issued >= 2) {
+                HIP_OK(dsmc_launch_phys(&rd->E, (const DsmcEnv *)rd->env_d.ptr, nullptr, rec, Q, p, late_grid,
+                                        sh.block, LATE_INLINE_SVC_BATCH, INT_MAX, st));
                 if (timed) HIP_OK(hipEventRecord(ev.back(), st));
             } else {
                 HIP_OK(dsmc_launch_flight(&rd->E, rec, Q, p, flight_grid, sh.block, sh.relaunch_batch,
@@ -373,6 +393,7 @@ dsmc_launch_shape default_shape(const DsmcEnv &E, int short_hops)
     s.block = 256;
     s.late_step_budget = 0;
     s.park_batch = 0;
+    s.late_inline = 0;
     if (E.mode == 0 && !E.advanced) {
         /* compat ballistic particles make hundreds of hops, many of one step:
          * with the service in its own kernel every hop is a round of launches
@@ -408,15 +429,18 @@ dsmc_launch_shape default_shape(const DsmcEnv &E, int short_hops)
         s.step_budget = INT_MAX;
         s.service_inline = 1;
     } else {
-        /* long hops (Ceres), 4e6 particles and more: the lean flight kernel.  1e7
-         * particles: 5 blocks per CU, a 4096-step budget in the first two rounds
-         * (1024 after them), refills of 12 idle lanes: 124 ms of flight kernel per
-         * run (refills of 4 / 8 / 16 / 32 lanes: 165 / 125 / 126 / 137 ms) */
+        /* long hops (Ceres), 4e6 particles and more: the lean flight kernel for
+         * the first two rounds (a 4096-step budget, 5 blocks per CU, refills of 8
+         * idle lanes), then the rest of the run in one launch with the landing
+         * service inline.  1e7 particles, flight kernels per run: 120.4 ms
+         * (refills of 12 / 16 lanes: 122.2 / 124.3 ms); 125.8 ms with rounds of
+         * the split kernels to the end (1024-step budgets after the second) */
         s.blocks_per_cu = 5;
-        s.relaunch_batch = 12;
+        s.relaunch_batch = 8;
         s.step_budget = 4096;
         s.service_inline = 0;
         s.late_step_budget = 1024;
+        s.late_inline = 1;
     }
     return s;
 }
@@ -426,7 +450,8 @@ bool shape_ok(const dsmc_launch_shape &s)
     return s.block >= 64 && s.block <= 256 && s.block % 64 == 0 && s.blocks_per_cu > 0 &&
            s.blocks_per_cu * s.block <= 2048 && s.relaunch_batch >= 1 && s.relaunch_batch <= 64 &&
            s.step_budget >= 16 && (s.service_inline == 0 || s.service_inline == 1) &&
-           (s.late_step_budget == 0 || s.late_step_budget >= 16) && s.park_batch >= 0 && s.park_batch <= 64;
+           (s.late_step_budget == 0 || s.late_step_budget >= 16) && s.park_batch >= 0 && s.park_batch <= 64 &&
+           (s.late_inline == 0 || s.late_inline == 1);
 }
 
 int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st)

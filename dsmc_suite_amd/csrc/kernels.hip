@@ -204,7 +204,10 @@ struct DsmcOpsStage {
 __device__ static inline void stage_init() { stg_cnt[threadIdx.x] = 0; }
 
 /* Issues the wavefront's staged deposits; every lane of the wavefront calls it.
- * List: slot 0 of every lane that has one, then slot 1. */
+ * List: slot 0 of every lane that has one, then slot 1.  LAZY (after a batch of
+ * cell logic, which stages at most one record per lane): only once some lane's
+ * slots are full or 16 records (one full atomic instruction) wait. */
+template <bool LAZY = false>
 __device__ static inline void stage_issue()
 {
     const int t = threadIdx.x, lane = (int)__lane_id();
@@ -212,6 +215,7 @@ __device__ static inline void stage_issue()
     const uint64_t m1 = __ballot(k >= 1);
     if (m1 == 0) return;
     const uint64_t m2 = __ballot(k >= 2);
+    if (LAZY && m2 == 0 && __popcll(m1) < 16) return;
     const int n1 = __popcll(m1), tot = n1 + __popcll(m2);
     const uint64_t lt = lanemask_lt();
     int32_t *idx = stg_idx[t >> 6];
@@ -468,7 +472,7 @@ lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batc
                         else lean_refresh(N, L, E, TB);
                     }
                 }
-                stage_issue();
+                stage_issue<true>();
             }
             uint64_t ended = __ballot(L.phase != LANE_FLIGHT && L.phase != LANE_IDLE);
             if (ended && (__popcll(ended | idle) >= refill_batch || __ballot(L.phase == LANE_FLIGHT) == 0)) break;
@@ -766,7 +770,7 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
                         }
                     }
                 }
-                if constexpr (STAGE) stage_issue();
+                if constexpr (STAGE) stage_issue<true>();
             }
           } else {
             /* two steps per vote, as in flight_kernel (+2-4% on pools, single

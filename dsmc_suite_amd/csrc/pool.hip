@@ -278,6 +278,7 @@ int pool_create(Pool *P, const PoolClass &c, int max_runs, int64_t max_slots, in
     sh.service_inline = 1;
     sh.late_step_budget = 0;
     sh.park_batch = 0;
+    sh.late_inline = 0;
     return pool_set_shape(P, shape ? *shape : sh);
 }
 

@@ -238,6 +238,10 @@ typedef struct dsmc_launch_shape {
                                  every round after the first two; 0 = step_budget (pools ignore it) */
     int park_batch;           /* physical ballistic flights: lanes whose step needs the cell logic
                                  wait until this many of a wavefront do (1-64); 0 = the default */
+    int late_inline;          /* single physical ballistic runs with the separate service kernel:
+                                 1 = after the first two rounds, the rest of the run in one launch
+                                 of the flight kernel with the landing service inline and no step
+                                 budget; 0 = rounds of the split kernels to the end (pools ignore it) */
 } dsmc_launch_shape;
 
 /* Staged runs: inputs uploaded once and kept resident in device memory, the run

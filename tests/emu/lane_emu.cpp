@@ -113,6 +113,68 @@ static void run_one_lean(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t b
     }
 }
 
+/* late_inline: a particle's first two flight launches are the event-batched
+ * flight kernel's (with the service kernel between them), every later one the
+ * flight kernel with the service inline, lean steps and no budget, which takes
+ * the particle over from the record the first two left (paused in flight with
+ * its open visit, or relaunched by the service kernel). */
+static void run_one_hybrid(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t budget)
+{
+    DsmcRec R;
+    memset(&R, 0x5A, sizeof R);
+    DsmcLane L;
+    garbage(L);
+    lane_start<DsmcOpsHost, 1, 0>(L, E, n, cnt);
+    if (L.phase != LANE_FLIGHT) return;
+    rec_store_all<1>(L, R, 0);
+    DsmcCellTabs TB = {E.tq, E.hr, E.cs};
+    const int64_t cap = E.max_steps > 0 ? E.max_steps : INT64_MAX;
+    for (int round = 0; round < 2; round++) {
+        DsmcLane F;
+        garbage(F);
+        rec_load_flight<1, 0>(F, R);
+        const int64_t end = F.steps + budget;
+        LeanLane N;
+        N.stop = end < cap ? end : cap;
+        N.pend = false;
+        lean_refresh(N, F, E, TB);
+        bool paused = false;
+        while (F.phase == LANE_FLIGHT) {
+            if (lane_fast_phys(F, E, N)) continue;
+            lean_tail<DsmcOpsHost>(F, E, TB, N);
+            if (F.phase != LANE_FLIGHT) break;
+            if (F.steps >= end) { paused = true; break; }
+            lean_refresh(N, F, E, TB);
+        }
+        if (!paused) lane_flush<DsmcOpsHost>(F, E);
+        rec_store_flight<1, 0>(F, R);
+        if (paused) continue;
+        DsmcLane S;
+        garbage(S);
+        rec_load_all<1>(S, R);
+        lane_service<DsmcOpsHost, 1, 0>(S, E, cnt);
+        if (S.phase == LANE_DONE) return;
+        rec_store_all<1>(S, R, 0);
+    }
+    DsmcLane F;
+    garbage(F);
+    rec_load_phys(F, R);
+    LeanLane N;
+    N.stop = cap;
+    N.pend = false;
+    lean_refresh(N, F, E, TB);
+    for (;;) {
+        if (lane_fast_phys(F, E, N)) continue;
+        lean_tail<DsmcOpsHost>(F, E, TB, N);
+        if (F.phase == LANE_FLIGHT) { lean_refresh(N, F, E, TB); continue; }
+        DsmcLane T = F;
+        lane_service<DsmcOpsHost, 1, 0>(T, E, cnt);
+        F = T;
+        if (F.phase == LANE_DONE) return;
+        lean_refresh(N, F, E, TB);
+    }
+}
+
 template <int MODE, int ADV>
 static void run_one_inline(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t budget)
 {
@@ -203,11 +265,13 @@ extern "C" int emu_advanced(const dsmc_params *p, const dsmc_surface *s, const d
 /* path 0: one particle at a time straight through the state machine; 1: the
  * split flight / service kernels' record traffic; 2: the physical flight kernel
  * with the service inline; 3: the event-batched physical flight kernel (lean
- * steps) with the service kernel.  `budget`: steps per flight launch (paths 1-3). */
+ * steps) with the service kernel; 4: path 3 for two launches, then the flight
+ * kernel with the service inline (late_inline).  `budget`: steps per flight
+ * launch (paths 1-3; path 4's first two launches). */
 extern "C" int emu_ballistic_path(const dsmc_params *p, const dsmc_surface *s, dsmc_result *r, int path,
                                   int64_t budget)
 {
-    if (path < 0 || path > 3 || (path && budget < 1) || (path == 3 && p->mode != DSMC_MODE_PHYSICAL)) return -1;
+    if (path < 0 || path > 4 || (path && budget < 1) || (path >= 3 && p->mode != DSMC_MODE_PHYSICAL)) return -1;
     DsmcEnv E;
     dsmc::HostTables T;
     int rc = dsmc::build_env(p, s, 0, &E, &T);
@@ -253,6 +317,7 @@ extern "C" int emu_ballistic_path(const dsmc_params *p, const dsmc_surface *s, d
         else if (path == 2 && p->mode == DSMC_MODE_COMPAT) run_one_inline<0, 0>(E, n, cnt, budget);
         else if (path == 2) run_one_inline<1, 0>(E, n, cnt, budget);
         else if (path == 3) run_one_lean(E, n, cnt, budget);
+        else if (path == 4) run_one_hybrid(E, n, cnt, budget);
         else if (p->mode == DSMC_MODE_COMPAT) run_one<0, 0>(L, E, n, cnt);
         else run_one<1, 0>(L, E, n, cnt);
     }

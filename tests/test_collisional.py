@@ -200,7 +200,7 @@ def test_histogram_count_identity(lib):
 
 
 # ---------------------------------------------------------------- lane state machine vs oracle (CPU)
-@pytest.mark.parametrize("path,budget", [(0, 0), (1, 23), (2, 23), (2, 4096)])
+@pytest.mark.parametrize("path,budget", [(0, 0), (1, 23), (2, 23), (2, 4096), (3, 23), (4, 23)])
 def test_phase1_histograms_through_record_paths(lib, emu, path, budget):
     o = orc(True)
     surf = surface(lib)

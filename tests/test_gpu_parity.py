@@ -247,19 +247,23 @@ def test_velocity_records_caps(lib):
     assert np.all(np.isfinite(v)) and np.all(np.linalg.norm(v, axis=1) < p.v_esc * 1.5)
 
 
-@pytest.mark.parametrize("budget,late,park", [(37, 23, 0), (4096, 1024, 0), (37, 23, 1), (37, 23, 64)])
-def test_default_split_path_matches_oracle(lib, dev, budget, late, park):
+@pytest.mark.parametrize("budget,late,park,linl", [(37, 23, 0, 0), (4096, 1024, 0, 0), (37, 23, 1, 0),
+                                                   (37, 23, 64, 0), (37, 23, 0, 1), (4096, 1024, 0, 1)])
+def test_default_split_path_matches_oracle(lib, dev, budget, late, park, linl):
     """The path a physical run of 4e6 particles or more takes by default
-    (flight_kernel<1,0> and the separate service kernel, 5 blocks per CU,
-    16-lane refills, deposits through the LDS slot table, velocity moments on,
-    a step budget in the first two rounds and a later one after them, open cell
-    visits carried in the records across launches), forced on 3000 Ceres
-    particles through the launch shape: per-particle results, deposit counts
-    and fate counters bit for bit against the oracle, moments to the
-    summation-order tolerance.  37/23-step launches make every visit cross many
-    launch boundaries; 4096/1024 are the default's own budgets.  Lanes whose step
-    needs the cell logic wait for the default batch of others, for none (1) and
-    for the whole wavefront (64)."""
+    (lean_flight_kernel and the separate service kernel, 5 blocks per CU,
+    16-lane refills, cell deposits staged in LDS and issued as per-cell records,
+    velocity moments on, a step budget in the first two rounds and a later one
+    after them, open cell visits carried in the records across launches),
+    forced on 3000 Ceres particles through the launch shape: per-particle
+    results, deposit counts and fate counters bit for bit against the oracle,
+    moments to the summation-order tolerance.  37/23-step launches make every
+    visit cross many launch boundaries; 4096/1024 are the default's own
+    budgets.  Lanes whose step needs the cell logic wait for the default batch
+    of others, for none (1) and for the whole wavefront (64).  linl = 1: the
+    rounds after the first two run as one launch of the kernel with the landing
+    service inline (late_inline), which takes over particles paused in flight
+    with their open visits and relaunched ones."""
     p = capi.preset(lib, PRESET_CERES_PHYSICAL, 36, 72, n=3000, seed=44)
     surf = icosphere(lib, 3)
     want = run_oracle(dev, p, surf, nthreads=8, moments=True)
@@ -267,14 +271,15 @@ def test_default_split_path_matches_oracle(lib, dev, budget, late, park):
     st = C.c_int(0)
     run = lib.dsmc_run_create(C.byref(p), C.byref(surf.surface), C.byref(got.result), C.byref(st))
     assert run and st.value == 0
-    sh = capi.LaunchShape(256, 5, 16, budget, 0, late, park)
-    print(f"split path: 3000 Ceres particles, budgets {budget}/{late}, park batch {park}", flush=True)
+    sh = capi.LaunchShape(256, 5, 16, budget, 0, late, park, linl)
+    print(f"split path: 3000 Ceres particles, budgets {budget}/{late}, park batch {park}, late inline {linl}",
+          flush=True)
     assert lib.dsmc_run_launch(run, C.byref(sh)) == 0 and lib.dsmc_run_sync(run) == 0
     assert lib.dsmc_run_collect(run, C.byref(got.result)) == 0
     rounds = C.c_int()
     assert lib.dsmc_run_timing(run, None, C.byref(rounds)) == 0
     lib.dsmc_run_destroy(run)
-    assert rounds.value >= (20 if budget < 100 else 3), rounds.value
+    assert rounds.value >= (3 if linl or budget >= 100 else 20), rounds.value
     for k in ("steps", "end_state", "fate", "hops", "vcount"):
         assert np.array_equal(getattr(got, k), getattr(want, k)), k
     assert got.counters() == want.counters()

@@ -248,15 +248,17 @@ def test_lane_state_machine_matches_oracle_physical(lib, emu):
 
 
 @pytest.mark.parametrize("body,lifetime", [(0, None), (3, None), (0, 1e30)])
-@pytest.mark.parametrize("budget", [23, 4096])
-def test_lean_flight_path_matches_oracle(lib, emu, body, lifetime, budget):
+@pytest.mark.parametrize("budget,path", [(23, 3), (4096, 3), (23, 4), (300, 4)])
+def test_lean_flight_path_matches_oracle(lib, emu, body, lifetime, budget, path):
     """The event-batched physical flight kernel's lane sequence (lean steps
     while nothing but the step count, the sunlit count and the open visit
     change, the full step logic when one needs it, the cell entries reloaded
     after it; records and the service kernel between launches): bit-identical
     to the oracle for a long-hop body (Ceres) and a short-hop one (Mercury),
     with velocity moments; and with a photolysis lifetime so long that the
-    lean steps' sunlit-count limit is clamped to INT32_MAX."""
+    lean steps' sunlit-count limit is clamped to INT32_MAX.  Path 4: two such
+    launches, then the kernel with the landing service inline takes each
+    particle over from the record they left (late_inline)."""
     from dsmc_suite_amd import workloads as W
     dev = oracle(devmath=True)
     b = W.BODIES[body]
@@ -267,7 +269,7 @@ def test_lean_flight_path_matches_oracle(lib, emu, body, lifetime, budget):
     want = run_oracle(dev, p, surf, nthreads=8, moments=True)
     got = Buffers(p, 5, moments=True)
     emu.emu_ballistic_path.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int64]
-    assert emu.emu_ballistic_path(C.byref(p), C.byref(surf.surface), C.byref(got.result), 3, budget) == 0
+    assert emu.emu_ballistic_path(C.byref(p), C.byref(surf.surface), C.byref(got.result), path, budget) == 0
     for k in ("steps", "end_state", "fate", "hops", "vcount", "horizon"):
         assert np.array_equal(getattr(got, k), getattr(want, k)), k
     assert got.counters() == want.counters()
