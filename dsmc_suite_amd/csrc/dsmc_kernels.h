@@ -55,6 +55,8 @@ hipError_t dsmc_launch_nsig(const double *ndens, const int32_t *valid, double si
 /* a physical run's per-cell records into vmom and vcount (no-op without records) */
 hipError_t dsmc_launch_fold(const DsmcEnv *E, int64_t cells, hipStream_t st);
 /* n zeros of 8 bytes (graph-safe clear) */
+/* counts of a pool run's 32-byte cell records into its vcount */
+hipError_t dsmc_launch_fold_counts(const double *acc, unsigned long long *vcount, int64_t cells, hipStream_t st);
 hipError_t dsmc_launch_fill0(void *p, size_t n8, hipStream_t st);
 hipError_t dsmc_launch_verlet(double *state, int n, int nsteps, double GM, double R,
                               double max_height, double h, double *traj, hipStream_t s);

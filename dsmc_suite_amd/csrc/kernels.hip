@@ -929,6 +929,16 @@ __global__ void __launch_bounds__(256) fold_cellacc_kernel(const double *acc, do
     }
 }
 
+/* Counts of cell records (fourth word, an exact integer below 2^53) into vcount. */
+__global__ void __launch_bounds__(256) fold_counts_kernel(const double *acc, unsigned long long *vcount, int64_t cells)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += stride) {
+        const double n = acc[4 * c + 3];
+        if (n != 0.0) vcount[c] += (unsigned long long)n;
+    }
+}
+
 /* ------------------------------------------------------------------ reset */
 __global__ void reset_kernel(unsigned int *ctl, int p, unsigned int *rounds)
 {
@@ -1136,6 +1146,16 @@ extern "C" hipError_t dsmc_launch_fold(const DsmcEnv *E, int64_t cells, hipStrea
     if (blocks > 2048) blocks = 2048;
     hipLaunchKernelGGL(fold_cellacc_kernel, dim3((unsigned)blocks), dim3(256), 0, st, (const double *)E->cellacc,
                        E->vmom, E->vcount, cells);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t dsmc_launch_fold_counts(const double *acc, unsigned long long *vcount, int64_t cells,
+                                               hipStream_t st)
+{
+    if (!acc || !vcount || cells <= 0) return hipErrorInvalidValue;
+    int64_t blocks = (cells + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(fold_counts_kernel, dim3((unsigned)blocks), dim3(256), 0, st, acc, vcount, cells);
     return hipGetLastError();
 }
 

@@ -21,6 +21,7 @@ struct PoolRun {
     int used = 0, done = 0, cancelled = 0;
     int kind = 0;              /* 0: ballistic (optionally keeping histograms), 1: collisional */
     int hist = -1, coll = -1;  /* histogram slot (phase 1) / fit-table slot (phase 2) */
+    int moments = 0;           /* the caller takes velocity moments (every run deposits cell records) */
     dsmc_params p;
     DsmcEnv E;
     int64_t slot0 = 0, n = 0;

@@ -472,9 +472,13 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out, int hist, 
     E.density = nullptr;
     E.density_from_count = 1;
     E.vcount = (unsigned long long *)P->vcount.ptr + (size_t)g * cells;
-    /* moments runs deposit into 32-byte cell records (vmom and vcount at collection) */
+    /* every run deposits into 32-byte cell records, staged and issued four lanes
+     * per record (vcount, and vmom for runs with moments, at collection): a
+     * density-only run deposits the same records and returns their counts (the
+     * 64-run batch: 8.4e10 particle-steps/s against 7.7e10 with one count
+     * atomic per visit) */
     E.vmom = nullptr;
-    E.cellacc = pr.moments ? (double *)P->cellacc.ptr + (size_t)g * cells * 4 : nullptr;
+    E.cellacc = (double *)P->cellacc.ptr + (size_t)g * cells * 4;
     E.horizon = (double *)P->horizon.ptr + (size_t)g * P->cls.ftd * P->cls.ffd;
     E.counters = (unsigned long long *)P->counters.ptr + (size_t)g * 8;
     E.ttime = nullptr;
@@ -532,6 +536,7 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out, int hist, 
     R.geo = geo;
     R.tag = tag;
     R.kind = pr.kind;
+    R.moments = pr.moments;
     R.hist = hist;
     R.coll = coll;
     R.first_seq = P->issued;
@@ -600,8 +605,10 @@ int pool_collect(Pool *P, int g, dsmc_result *r)
     HIP_OK(hipStreamWaitEvent(cs, P->last_ev, 0));
     char *o = (char *)P->out_h;
     unsigned long long cnt[8];
+    /* a density-only run's counts: folded from its records on the device first */
+    if (!R.moments) HIP_OK(dsmc_launch_fold_counts(E.cellacc, E.vcount, (int64_t)cells, cs));
     HIP_OK(hipMemcpyAsync(o, E.vcount, cells * 8, hipMemcpyDeviceToHost, cs));
-    if (E.cellacc) HIP_OK(hipMemcpyAsync(o + cells * 8, E.cellacc, cells * 32, hipMemcpyDeviceToHost, cs));
+    if (R.moments) HIP_OK(hipMemcpyAsync(o + cells * 8, E.cellacc, cells * 32, hipMemcpyDeviceToHost, cs));
     HIP_OK(hipMemcpyAsync(o + cells * 40, E.horizon, nh * 8, hipMemcpyDeviceToHost, cs));
     HIP_OK(hipMemcpyAsync(o + cells * 40 + nh * 8, E.counters, sizeof cnt, hipMemcpyDeviceToHost, cs));
     HIP_OK(hipStreamSynchronize(cs));
@@ -610,11 +617,11 @@ int pool_collect(Pool *P, int g, dsmc_result *r)
     const double *acc = (const double *)(o + cells * 8);
     /* a moments run's counts and velocity sums are in its cell records: exact
      * integer counts (below 2^53) in the fourth word */
-    if (E.cellacc) for (size_t i = 0; i < cells; i++) vc[i] += (int64_t)acc[4 * i + 3];
+    if (R.moments) for (size_t i = 0; i < cells; i++) vc[i] += (int64_t)acc[4 * i + 3];
     if (r->vcount) memcpy(r->vcount, vc, cells * 8);
     if (r->density) for (size_t i = 0; i < cells; i++) r->density[i] = (double)vc[i];
     if (r->vmom) {
-        if (E.cellacc) {
+        if (R.moments) {
             for (size_t i = 0; i < cells; i++)
                 for (int k = 0; k < 3; k++) r->vmom[3 * i + k] = acc[4 * i + k];
         } else {
