@@ -232,6 +232,8 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.dsmc_run_kernel_ms.restype = C.c_double
     lib.dsmc_run_collect.argtypes = [C.c_void_p, R]
     lib.dsmc_run_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    lib.dsmc_run_lean_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int64)]
     lib.dsmc_run_destroy.argtypes = [C.c_void_p]
     lib.dsmc_run_destroy.restype = None
     lib.dsmc_verlet_steps.argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,

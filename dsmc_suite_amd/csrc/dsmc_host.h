@@ -84,6 +84,10 @@ struct RunDev {
     bool ctl_pinned = false;
     int rounds = 0;
     double flight_ms = 0.0;
+    /* the event-batched flight kernel's share of the last launch */
+    int lean_launches = 0;
+    double lean_ms = 0.0;
+    int64_t lean_steps = 0;
     int short_hops = 0;                   /* physical: landings every few dozen steps */
     void drop_buffers();                  /* every buffer released: a reused RunDev starts empty */
     ~RunDev();

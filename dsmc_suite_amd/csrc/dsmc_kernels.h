@@ -7,7 +7,8 @@
 
 /* Slot queues of a set of runs.  ctl (unsigned int):
  *   [0] fly_count[0] [1] fly_head[0] [2] fly_count[1] [3] fly_head[1]
- *   [4] svc_count    [8] rounds executed */
+ *   [4] svc_count    [8] rounds executed
+ *   [12..13] particle-steps taken by the event-batched flight kernel (uint64) */
 struct DsmcQueues {
     int32_t *fly[2];
     int32_t *svc;

@@ -306,6 +306,11 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
     HIP_OK(hipMemcpyAsync(rd->env_d.ptr, &rd->E, sizeof(DsmcEnv), hipMemcpyHostToDevice, st));
     rd->rounds = 0;
     rd->flight_ms = 0.0;
+    rd->lean_launches = 0;
+    rd->lean_ms = 0.0;
+    rd->lean_steps = 0;
+    const bool lean = !inline_svc && rd->E.mode == DSMC_MODE_PHYSICAL && !rd->E.advanced;
+    std::vector<char> is_lean;          /* per timed flight launch */
     if (n == 0) return DSMC_OK;
     int64_t blocks = (n + 255) / 256;
     int start_grid = (int)(blocks < (int64_t)cus * 8 ? blocks : (int64_t)cus * 8);
@@ -349,6 +354,7 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
                 HIP_OK(hipEventCreate(&b));
                 ev.push_back(b);
                 HIP_OK(hipEventRecord(a, st));
+                is_lean.push_back(0);
             }
             if (inline_svc) {
                 HIP_OK(dsmc_launch_phys(&rd->E, (const DsmcEnv *)rd->env_d.ptr, nullptr, rec, Q, p, flight_grid,
@@ -362,6 +368,7 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
                 HIP_OK(dsmc_launch_flight(&rd->E, rec, Q, p, flight_grid, sh.block, sh.relaunch_batch,
                                           issued < 2 ? sh_budget : late_budget, sh.park_batch, st));
                 if (timed) HIP_OK(hipEventRecord(ev.back(), st));
+                if (timed) is_lean.back() = lean ? 1 : 0;
                 HIP_OK(dsmc_launch_service(&rd->E, rec, Q, p, service_grid, st));
             }
             p ^= 1;
@@ -379,10 +386,15 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
     /* physical runs with moments deposited into per-cell records: into vmom, vcount */
     HIP_OK(dsmc_launch_fold(&rd->E, (int64_t)rd->E.hd * rd->E.ftd * rd->E.ffd, st));
     rd->rounds = (int)rd->host_ctl[8];
+    rd->lean_steps = (int64_t)((uint64_t)rd->host_ctl[12] | ((uint64_t)rd->host_ctl[13] << 32));
     for (size_t i = 0; i + 1 < ev.size(); i += 2) {
         float ms = 0.f;
         HIP_OK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
         rd->flight_ms += ms;
+        if (is_lean[i / 2]) {
+            rd->lean_ms += ms;
+            rd->lean_launches++;
+        }
     }
     return DSMC_OK;
 }
@@ -774,6 +786,15 @@ extern "C" int dsmc_run_timing(dsmc_run *run, double *flight_ms, int *rounds)
     if (!run || !run->launched) return DSMC_ERR_ARG;
     if (flight_ms) *flight_ms = run->rd.flight_ms;
     if (rounds) *rounds = run->rd.rounds;
+    return DSMC_OK;
+}
+
+extern "C" int dsmc_run_lean_timing(dsmc_run *run, double *lean_ms, int *launches, int64_t *steps)
+{
+    if (!run || !run->launched) return DSMC_ERR_ARG;
+    if (lean_ms) *lean_ms = run->rd.lean_ms;
+    if (launches) *launches = run->rd.lean_launches;
+    if (steps) *steps = run->rd.lean_steps;
     return DSMC_OK;
 }
 

@@ -428,6 +428,9 @@ lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batc
     int32_t slot = -1;
     int64_t budget_end = 0;
     bool exhausted = false;
+    /* particle-steps taken in this launch, modulo 2^32 per lane (each particle's
+     * steps at its store minus its steps at its load) */
+    uint32_t taken = 0;
     for (;;) {
         uint64_t idle = __ballot(L.phase == LANE_IDLE);
         uint64_t flying = ~idle & __ballot(true);
@@ -442,6 +445,7 @@ lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batc
                 if (qi < nq) {
                     slot = flyq[qi];
                     rec_load_flight<1, 0>(L, rec[slot]);
+                    taken -= (uint32_t)L.steps;
                     budget_end = step_budget > 0 ? L.steps + step_budget : INT64_MAX;
                     N.stop = budget_end < cap ? budget_end : cap;
                     N.pend = false;
@@ -483,12 +487,17 @@ lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batc
             if (hop_end) lane_flush<DsmcOpsStage>(L, E);   /* a pause keeps the open visit */
             if (paused) L.phase = LANE_FLIGHT;
             rec_store_flight<1, 0>(L, rec[slot]);
+            taken += (uint32_t)L.steps;
         }
         stage_issue();
         wave_append(hop_end, slot, Q.svc, &Q.ctl[4]);
         wave_append(paused, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
         if (hop_end || paused) L.phase = LANE_IDLE;
     }
+    /* the launch's particle-steps into ctl[12..13] (64-bit, per run) */
+    unsigned long long wsum = taken;
+    for (int o = 32; o > 0; o >>= 1) wsum += __shfl_xor(wsum, o);
+    if (lane == 0 && wsum) atomicAdd((unsigned long long *)&Q.ctl[12], wsum);
 }
 
 /* Flight kernel of a pool: as above, with each lane's run constants in registers

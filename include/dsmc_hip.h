@@ -257,6 +257,10 @@ int dsmc_run_sync(dsmc_run *run);
 double dsmc_run_kernel_ms(dsmc_run *run);        /* HIP-event time of the last launch */
 /* Device time spent in the flight kernel and number of rounds of the last launch. */
 int dsmc_run_timing(dsmc_run *run, double *flight_ms, int *rounds);
+/* The event-batched flight kernel's share of the last launch (physical ballistic
+ * runs with the separate service kernel): its device time, its launches and the
+ * particle-steps it took, counted on the device (0 when the run took none). */
+int dsmc_run_lean_timing(dsmc_run *run, double *lean_ms, int *launches, int64_t *steps);
 int dsmc_run_collect(dsmc_run *run, dsmc_result *r);
 void dsmc_run_destroy(dsmc_run *run);
 

@@ -278,8 +278,17 @@ def test_default_split_path_matches_oracle(lib, dev, budget, late, park, linl):
     assert lib.dsmc_run_collect(run, C.byref(got.result)) == 0
     rounds = C.c_int()
     assert lib.dsmc_run_timing(run, None, C.byref(rounds)) == 0
+    lean_ms, lean_n, lean_steps = C.c_double(), C.c_int(), C.c_int64()
+    assert lib.dsmc_run_lean_timing(run, C.byref(lean_ms), C.byref(lean_n), C.byref(lean_steps)) == 0
     lib.dsmc_run_destroy(run)
     assert rounds.value >= (3 if linl or budget >= 100 else 20), rounds.value
+    # the event-batched flight kernel's own step count: every step of the run
+    # without the late inline launch, the first two rounds' steps with it
+    total = int(want.steps.sum())
+    if linl:
+        assert lean_n.value == 2 and 0 < lean_steps.value <= total, (lean_n.value, lean_steps.value, total)
+    else:
+        assert lean_n.value == rounds.value and lean_steps.value == total, (lean_n.value, lean_steps.value)
     for k in ("steps", "end_state", "fate", "hops", "vcount"):
         assert np.array_equal(getattr(got, k), getattr(want, k)), k
     assert got.counters() == want.counters()
