@@ -327,12 +327,13 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
      * 8192-20000 in the later rounds) */
     const int64_t sh_budget = sh.step_budget;
     const int64_t late_budget = sh.late_step_budget > 0 ? sh.late_step_budget : sh_budget;
-    /* late_inline: the rounds after the first two run as launches of the kernel
+    /* late_inline = k: the rounds after the first k run as launches of the kernel
      * with the landing service inline and no step budget, so the particles still
      * flying (most of them multi-hop, each hop of which would otherwise wait for
      * a round's end) finish in the first such launch; it holds fewer blocks per
      * CU than the lean flight kernel */
     const bool late_inline = !inline_svc && sh.late_inline && rd->E.mode == DSMC_MODE_PHYSICAL && !rd->E.advanced;
+    const int split_rounds = sh.late_inline;
     int late_grid = flight_grid;
     if (late_inline) {
         const int occ = dsmc_flight_occupancy(&rd->E, 1, sh.block);
@@ -360,7 +361,7 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
                 HIP_OK(dsmc_launch_phys(&rd->E, (const DsmcEnv *)rd->env_d.ptr, nullptr, rec, Q, p, flight_grid,
                                         sh.block, sh.relaunch_batch, sh_budget, st));
                 if (timed) HIP_OK(hipEventRecord(ev.back(), st));
-            } else if (late_inline && issued >= 2) {
+            } else if (late_inline && issued >= split_rounds) {
                 HIP_OK(dsmc_launch_phys(&rd->E, (const DsmcEnv *)rd->env_d.ptr, nullptr, rec, Q, p, late_grid,
                                         sh.block, LATE_INLINE_SVC_BATCH, INT_MAX, st));
                 if (timed) HIP_OK(hipEventRecord(ev.back(), st));
@@ -452,7 +453,7 @@ dsmc_launch_shape default_shape(const DsmcEnv &E, int short_hops)
         s.step_budget = 4096;
         s.service_inline = 0;
         s.late_step_budget = 1024;
-        s.late_inline = 1;
+        s.late_inline = 2;
     }
     return s;
 }
@@ -463,7 +464,7 @@ bool shape_ok(const dsmc_launch_shape &s)
            s.blocks_per_cu * s.block <= 2048 && s.relaunch_batch >= 1 && s.relaunch_batch <= 64 &&
            s.step_budget >= 16 && (s.service_inline == 0 || s.service_inline == 1) &&
            (s.late_step_budget == 0 || s.late_step_budget >= 16) && s.park_batch >= 0 && s.park_batch <= 64 &&
-           (s.late_inline == 0 || s.late_inline == 1);
+           s.late_inline >= 0 && s.late_inline <= 64;
 }
 
 int collect_run(RunDev *rd, const dsmc_params *p, dsmc_result *r, hipStream_t st)

@@ -239,7 +239,7 @@ typedef struct dsmc_launch_shape {
     int park_batch;           /* physical ballistic flights: lanes whose step needs the cell logic
                                  wait until this many of a wavefront do (1-64); 0 = the default */
     int late_inline;          /* single physical ballistic runs with the separate service kernel:
-                                 1 = after the first two rounds, the rest of the run in one launch
+                                 k = 1-64: after the first k rounds, the rest of the run in one launch
                                  of the flight kernel with the landing service inline and no step
                                  budget; 0 = rounds of the split kernels to the end (pools ignore it) */
 } dsmc_launch_shape;

@@ -248,7 +248,8 @@ def test_velocity_records_caps(lib):
 
 
 @pytest.mark.parametrize("budget,late,park,linl", [(37, 23, 0, 0), (4096, 1024, 0, 0), (37, 23, 1, 0),
-                                                   (37, 23, 64, 0), (37, 23, 0, 1), (4096, 1024, 0, 1)])
+                                                   (37, 23, 64, 0), (37, 23, 0, 2), (4096, 1024, 0, 2),
+                                                   (37, 23, 0, 5)])
 def test_default_split_path_matches_oracle(lib, dev, budget, late, park, linl):
     """The path a physical run of 4e6 particles or more takes by default
     (lean_flight_kernel and the separate service kernel, 5 blocks per CU,
@@ -260,8 +261,8 @@ def test_default_split_path_matches_oracle(lib, dev, budget, late, park, linl):
     moments to the summation-order tolerance.  37/23-step launches make every
     visit cross many launch boundaries; 4096/1024 are the default's own
     budgets.  Lanes whose step needs the cell logic wait for the default batch
-    of others, for none (1) and for the whole wavefront (64).  linl = 1: the
-    rounds after the first two run as one launch of the kernel with the landing
+    of others, for none (1) and for the whole wavefront (64).  linl = k: the
+    rounds after the first k run as one launch of the kernel with the landing
     service inline (late_inline), which takes over particles paused in flight
     with their open visits and relaunched ones."""
     p = capi.preset(lib, PRESET_CERES_PHYSICAL, 36, 72, n=3000, seed=44)
@@ -286,7 +287,7 @@ def test_default_split_path_matches_oracle(lib, dev, budget, late, park, linl):
     # without the late inline launch, the first two rounds' steps with it
     total = int(want.steps.sum())
     if linl:
-        assert lean_n.value == 2 and 0 < lean_steps.value <= total, (lean_n.value, lean_steps.value, total)
+        assert lean_n.value == linl and 0 < lean_steps.value <= total, (lean_n.value, lean_steps.value, total)
     else:
         assert lean_n.value == rounds.value and lean_steps.value == total, (lean_n.value, lean_steps.value)
     for k in ("steps", "end_state", "fate", "hops", "vcount"):
