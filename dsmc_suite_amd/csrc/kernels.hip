@@ -400,7 +400,7 @@ flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch, in
 #endif
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch, int64_t step_budget,
-                   int pend_batch)
+                   int pend_batch, int hop_to_fly)
 {
     extern __shared__ double lds_tabs[];
     DsmcCellTabs TB;
@@ -490,8 +490,14 @@ lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batc
             taken += (uint32_t)L.steps;
         }
         stage_issue();
-        wave_append(hop_end, slot, Q.svc, &Q.ctl[4]);
-        wave_append(paused, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
+        if (hop_to_fly) {
+            /* the next launch services landings itself (late_inline): the
+             * particles whose hop ended join the paused ones in its queue */
+            wave_append(hop_end || paused, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
+        } else {
+            wave_append(hop_end, slot, Q.svc, &Q.ctl[4]);
+            wave_append(paused, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
+        }
         if (hop_end || paused) L.phase = LANE_IDLE;
     }
     /* the launch's particle-steps into ctl[12..13] (64-bit, per run) */
@@ -744,7 +750,7 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
                     }
                 }
             }
-            flying = __ballot(L.phase == LANE_FLIGHT);
+            flying = __ballot(L.phase != LANE_IDLE);    /* lanes with work: a loaded particle may wait for its service */
         }
         if (flying == 0) {
             if (exhausted) break;
@@ -1038,16 +1044,25 @@ extern "C" hipError_t dsmc_launch_start(const DsmcEnv *E, int32_t run_id, DsmcRe
 }
 
 extern "C" hipError_t dsmc_launch_flight(const DsmcEnv *E, DsmcRec *rec, DsmcQueues Q, int p, int grid, int block,
-                                         int refill_batch, int64_t step_budget, int park_batch, hipStream_t st)
+                                         int refill_batch, int64_t step_budget, int park_batch, int hop_to_fly,
+                                         hipStream_t st)
 {
     if (!block_ok(block)) return hipErrorInvalidValue;
     const int mode = E->mode, adv = E->advanced;
     const size_t lds = sizeof(double) * (size_t)((E->ftd + 1) + (E->hd + 2) + 2 * (E->ffd + 1));
     if (mode == 1 && !adv)
         hipLaunchKernelGGL(lean_flight_kernel, dim3(grid), dim3(block), lds, st, *E, rec, Q, p, refill_batch,
-                           step_budget, park_batch > 0 ? park_batch : LEAN_PARK_BATCH);
+                           step_budget, park_batch > 0 ? park_batch : LEAN_PARK_BATCH, hop_to_fly);
+    else if (hop_to_fly)
+        return hipErrorInvalidValue;
     else
         DISPATCH(flight_kernel, dim3(grid), dim3(block), lds, st, *E, rec, Q, p, refill_batch, step_budget);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t dsmc_launch_reset(DsmcQueues Q, int p, hipStream_t st)
+{
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, Q.ctl, p, Q.ctl + 8);
     return hipGetLastError();
 }
 

@@ -114,10 +114,11 @@ static void run_one_lean(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t b
 }
 
 /* late_inline: a particle's first two flight launches are the event-batched
- * flight kernel's (with the service kernel between them), every later one the
+ * flight kernel's (the service kernel after the first), every later one the
  * flight kernel with the service inline, lean steps and no budget, which takes
  * the particle over from the record the first two left (paused in flight with
- * its open visit, or relaunched by the service kernel). */
+ * its open visit, relaunched by the service kernel, or with its hop ended in
+ * the second launch, which leaves its landings to the inline launch). */
 static void run_one_hybrid(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t budget)
 {
     DsmcRec R;
@@ -148,7 +149,7 @@ static void run_one_hybrid(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t
         }
         if (!paused) lane_flush<DsmcOpsHost>(F, E);
         rec_store_flight<1, 0>(F, R);
-        if (paused) continue;
+        if (paused || round == 1) continue;
         DsmcLane S;
         garbage(S);
         rec_load_all<1>(S, R);
@@ -164,7 +165,15 @@ static void run_one_hybrid(const DsmcEnv &E, int64_t n, DsmcCounts &cnt, int64_t
     N.pend = false;
     lean_refresh(N, F, E, TB);
     for (;;) {
-        if (lane_fast_phys(F, E, N)) continue;
+        if (F.phase == LANE_FLIGHT && lane_fast_phys(F, E, N)) continue;
+        if (F.phase != LANE_FLIGHT) {      /* a hop that ended in the second launch */
+            DsmcLane T = F;
+            lane_service<DsmcOpsHost, 1, 0>(T, E, cnt);
+            F = T;
+            if (F.phase == LANE_DONE) return;
+            lean_refresh(N, F, E, TB);
+            continue;
+        }
         lean_tail<DsmcOpsHost>(F, E, TB, N);
         if (F.phase == LANE_FLIGHT) { lean_refresh(N, F, E, TB); continue; }
         DsmcLane T = F;
