@@ -900,19 +900,63 @@ compat_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcRec *rec, DsmcQueues Q, int p, 
 }
 
 /* ------------------------------------------------------------------ service */
-/* One lane per queued slot.  POOL: the record names its run's environment. */
+/* Slots a lane holds for one wavefront-wide append (one atomic on the queue's
+ * counter per flush instead of one per wavefront iteration: a single address
+ * serves ~88M atomics/s on MI355X, and the service of a 1e7-particle round
+ * would otherwise issue one per 64 of its ~1e7 slots). */
+struct AppendBuf {
+    int32_t s0, s1, s2, s3;
+    int n;
+};
+
+__device__ static inline void append_flush(AppendBuf &A, int32_t *q, unsigned int *cnt)
+{
+    const uint64_t m0 = __ballot(A.n > 0);
+    if (m0 == 0) return;
+    const uint64_t m1 = __ballot(A.n > 1), m2 = __ballot(A.n > 2), m3 = __ballot(A.n > 3);
+    const unsigned int c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2), c3 = __popcll(m3);
+    const int leader = __ffsll((long long)m0) - 1;
+    unsigned int base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(cnt, c0 + c1 + c2 + c3);
+    base = __shfl(base, leader);
+    const uint64_t lt = lanemask_lt();
+    if (A.n > 0) q[base + __popcll(m0 & lt)] = A.s0;
+    if (A.n > 1) q[base + c0 + __popcll(m1 & lt)] = A.s1;
+    if (A.n > 2) q[base + c0 + c1 + __popcll(m2 & lt)] = A.s2;
+    if (A.n > 3) q[base + c0 + c1 + c2 + __popcll(m3 & lt)] = A.s3;
+    A.n = 0;
+}
+
+__device__ static inline void append_push(AppendBuf &A, bool pred, int32_t slot, int32_t *q, unsigned int *cnt)
+{
+    if (pred) {
+        if (A.n == 0) A.s0 = slot;
+        else if (A.n == 1) A.s1 = slot;
+        else if (A.n == 2) A.s2 = slot;
+        else A.s3 = slot;
+        A.n++;
+    }
+    if (__ballot(A.n == 4)) append_flush(A, q, cnt);
+}
+
+/* One lane per queued slot.  POOL: the record names its run's environment.  A
+ * single run's fate counters gather over the whole launch (one atomic per
+ * counter and wavefront at its end); relaunched slots are appended four per
+ * lane at a time. */
 template <int MODE, int ADV, bool POOL>
 __global__ void __launch_bounds__(256)
 service_kernel(DsmcEnv E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int p)
 {
     const unsigned int n = Q.ctl[4];
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    AppendBuf A;
+    A.n = 0;
+    DsmcCounts cnt = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int64_t b = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < (int64_t)n; b += stride) {
         int64_t i = b + __lane_id();
         bool fly = false, fin = false;
         int32_t slot = 0;
         unsigned long long *ctr = nullptr;
-        DsmcCounts cnt = {0, 0, 0, 0, 0, 0, 0, 0};
         if (i < n) {
             slot = Q.svc[i];
             int run = POOL ? rec[slot].run : 0;
@@ -925,9 +969,11 @@ service_kernel(DsmcEnv E0, const DsmcEnv *envs, DsmcRec *rec, DsmcQueues Q, int 
             ctr = E.counters;
             if (fly) rec_store_all<MODE>(L, rec[slot], run);
         }
-        if (__ballot(fin)) counts_flush_wave(cnt, ctr, fin);
-        wave_append(fly, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
+        if (POOL && __ballot(fin)) counts_flush_wave(cnt, ctr, fin);
+        append_push(A, fly, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
     }
+    append_flush(A, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
+    if (!POOL) counts_flush_wave(cnt, E0.counters, true);
 }
 
 __global__ void fill0_kernel(unsigned long long *p, size_t n)
