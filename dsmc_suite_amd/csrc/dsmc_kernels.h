@@ -34,10 +34,10 @@ hipError_t dsmc_launch_start(const DsmcEnv *E, int32_t run_id, DsmcRec *rec, int
                              DsmcQueues Q, int p, int grid, hipStream_t st);
 /* flight kernel of round p (one run) */
 hipError_t dsmc_launch_flight(const DsmcEnv *E, DsmcRec *rec, DsmcQueues Q, int p, int grid, int block,
-                              int refill_batch, int64_t step_budget, int park_batch, int hop_to_fly,
-                              hipStream_t st);
-/* the queue reset of round p alone (a round whose landings the next launch services) */
-hipError_t dsmc_launch_reset(DsmcQueues Q, int p, hipStream_t st);
+                              int refill_batch, int64_t step_budget, int park_batch, hipStream_t st);
+/* in place of the service kernel of round p: its service queue appended to the
+ * next fly queue (for a launch that services landings itself), then the reset */
+hipError_t dsmc_launch_handover(DsmcQueues Q, int p, int grid, hipStream_t st);
 /* service kernel of round p (one run), then the queue reset */
 extern "C" int dsmc_flight_occupancy(const DsmcEnv *E, int inline_svc, int block);
 hipError_t dsmc_launch_service(const DsmcEnv *E, DsmcRec *rec, DsmcQueues Q, int p, int grid, hipStream_t st);

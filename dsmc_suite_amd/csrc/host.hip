@@ -366,13 +366,13 @@ int launch_run(RunDev *rd, const dsmc_launch_shape *shape, hipStream_t st, int t
                                         sh.block, LATE_INLINE_SVC_BATCH, INT_MAX, st));
                 if (timed) HIP_OK(hipEventRecord(ev.back(), st));
             } else {
-                /* the round before the late inline launch leaves its landings to it */
-                const int hop_to_fly = late_inline && issued == split_rounds - 1;
                 HIP_OK(dsmc_launch_flight(&rd->E, rec, Q, p, flight_grid, sh.block, sh.relaunch_batch,
-                                          issued < 2 ? sh_budget : late_budget, sh.park_batch, hop_to_fly, st));
+                                          issued < 2 ? sh_budget : late_budget, sh.park_batch, st));
                 if (timed) HIP_OK(hipEventRecord(ev.back(), st));
                 if (timed) is_lean.back() = lean ? 1 : 0;
-                if (hop_to_fly) HIP_OK(dsmc_launch_reset(Q, p, st));
+                /* the round before the late inline launch leaves its landings to
+                 * it: they queue behind the paused particles, which start first */
+                if (late_inline && issued == split_rounds - 1) HIP_OK(dsmc_launch_handover(Q, p, service_grid, st));
                 else HIP_OK(dsmc_launch_service(&rd->E, rec, Q, p, service_grid, st));
             }
             p ^= 1;

@@ -263,6 +263,45 @@ __device__ static inline void counts_flush_wave(DsmcCounts &cnt, unsigned long l
     cnt.esc = cnt.frozen = cnt.photo = cnt.depleted = cnt.capped = cnt.coll = cnt.steps = cnt.done = 0;
 }
 
+/* Slots a lane holds for one wavefront-wide append (one atomic on the queue's
+ * counter per flush instead of one per wavefront iteration: a single address
+ * serves ~88M atomics/s on MI355X, and the service of a 1e7-particle round
+ * would otherwise issue one per 64 of its ~1e7 slots). */
+struct AppendBuf {
+    int32_t s0, s1, s2, s3;
+    int n;
+};
+
+__device__ static inline void append_flush(AppendBuf &A, int32_t *q, unsigned int *cnt)
+{
+    const uint64_t m0 = __ballot(A.n > 0);
+    if (m0 == 0) return;
+    const uint64_t m1 = __ballot(A.n > 1), m2 = __ballot(A.n > 2), m3 = __ballot(A.n > 3);
+    const unsigned int c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2), c3 = __popcll(m3);
+    const int leader = __ffsll((long long)m0) - 1;
+    unsigned int base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(cnt, c0 + c1 + c2 + c3);
+    base = __shfl(base, leader);
+    const uint64_t lt = lanemask_lt();
+    if (A.n > 0) q[base + __popcll(m0 & lt)] = A.s0;
+    if (A.n > 1) q[base + c0 + __popcll(m1 & lt)] = A.s1;
+    if (A.n > 2) q[base + c0 + c1 + __popcll(m2 & lt)] = A.s2;
+    if (A.n > 3) q[base + c0 + c1 + c2 + __popcll(m3 & lt)] = A.s3;
+    A.n = 0;
+}
+
+__device__ static inline void append_push(AppendBuf &A, bool pred, int32_t slot, int32_t *q, unsigned int *cnt)
+{
+    if (pred) {
+        if (A.n == 0) A.s0 = slot;
+        else if (A.n == 1) A.s1 = slot;
+        else if (A.n == 2) A.s2 = slot;
+        else A.s3 = slot;
+        A.n++;
+    }
+    if (__ballot(A.n == 4)) append_flush(A, q, cnt);
+}
+
 /* ------------------------------------------------------------------ start */
 /* Starts particles [0, n) of one run in slots slot0..slot0+n-1. */
 template <int MODE, int ADV>
@@ -271,6 +310,8 @@ start_kernel(DsmcEnv E, int32_t run_id, DsmcRec *rec, int64_t slot0, int64_t n, 
              unsigned int *fly_count)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    AppendBuf A;
+    A.n = 0;
     for (int64_t b = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < n; b += stride) {
         int64_t i = b + __lane_id();
         bool fly = false;
@@ -283,8 +324,9 @@ start_kernel(DsmcEnv E, int32_t run_id, DsmcRec *rec, int64_t slot0, int64_t n, 
             fly = L.phase == LANE_FLIGHT;
             if (fly) rec_store_all<MODE>(L, rec[slot], run_id);
         }
-        wave_append(fly, (int32_t)slot, flyq, fly_count);
+        append_push(A, fly, (int32_t)slot, flyq, fly_count);
     }
+    append_flush(A, flyq, fly_count);
 }
 
 /* ------------------------------------------------------------------ flight */
@@ -400,7 +442,7 @@ flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch, in
 #endif
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch, int64_t step_budget,
-                   int pend_batch, int hop_to_fly)
+                   int pend_batch)
 {
     extern __shared__ double lds_tabs[];
     DsmcCellTabs TB;
@@ -490,14 +532,8 @@ lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batc
             taken += (uint32_t)L.steps;
         }
         stage_issue();
-        if (hop_to_fly) {
-            /* the next launch services landings itself (late_inline): the
-             * particles whose hop ended join the paused ones in its queue */
-            wave_append(hop_end || paused, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
-        } else {
-            wave_append(hop_end, slot, Q.svc, &Q.ctl[4]);
-            wave_append(paused, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
-        }
+        wave_append(hop_end, slot, Q.svc, &Q.ctl[4]);
+        wave_append(paused, slot, Q.fly[1 - p], &Q.ctl[2 * (1 - p)]);
         if (hop_end || paused) L.phase = LANE_IDLE;
     }
     /* the launch's particle-steps into ctl[12..13] (64-bit, per run) */
@@ -900,45 +936,6 @@ compat_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcRec *rec, DsmcQueues Q, int p, 
 }
 
 /* ------------------------------------------------------------------ service */
-/* Slots a lane holds for one wavefront-wide append (one atomic on the queue's
- * counter per flush instead of one per wavefront iteration: a single address
- * serves ~88M atomics/s on MI355X, and the service of a 1e7-particle round
- * would otherwise issue one per 64 of its ~1e7 slots). */
-struct AppendBuf {
-    int32_t s0, s1, s2, s3;
-    int n;
-};
-
-__device__ static inline void append_flush(AppendBuf &A, int32_t *q, unsigned int *cnt)
-{
-    const uint64_t m0 = __ballot(A.n > 0);
-    if (m0 == 0) return;
-    const uint64_t m1 = __ballot(A.n > 1), m2 = __ballot(A.n > 2), m3 = __ballot(A.n > 3);
-    const unsigned int c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2), c3 = __popcll(m3);
-    const int leader = __ffsll((long long)m0) - 1;
-    unsigned int base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(cnt, c0 + c1 + c2 + c3);
-    base = __shfl(base, leader);
-    const uint64_t lt = lanemask_lt();
-    if (A.n > 0) q[base + __popcll(m0 & lt)] = A.s0;
-    if (A.n > 1) q[base + c0 + __popcll(m1 & lt)] = A.s1;
-    if (A.n > 2) q[base + c0 + c1 + __popcll(m2 & lt)] = A.s2;
-    if (A.n > 3) q[base + c0 + c1 + c2 + __popcll(m3 & lt)] = A.s3;
-    A.n = 0;
-}
-
-__device__ static inline void append_push(AppendBuf &A, bool pred, int32_t slot, int32_t *q, unsigned int *cnt)
-{
-    if (pred) {
-        if (A.n == 0) A.s0 = slot;
-        else if (A.n == 1) A.s1 = slot;
-        else if (A.n == 2) A.s2 = slot;
-        else A.s3 = slot;
-        A.n++;
-    }
-    if (__ballot(A.n == 4)) append_flush(A, q, cnt);
-}
-
 /* One lane per queued slot.  POOL: the record names its run's environment.  A
  * single run's fate counters gather over the whole launch (one atomic per
  * counter and wavefront at its end); relaunched slots are appended four per
@@ -1012,9 +1009,23 @@ __global__ void __launch_bounds__(256) fold_counts_kernel(const double *acc, uns
 }
 
 /* ------------------------------------------------------------------ reset */
-__global__ void reset_kernel(unsigned int *ctl, int p, unsigned int *rounds)
+/* The round before a launch with the landing service inline: the slots whose hop
+ * ended (the service queue) join the next round's fly queue behind the paused
+ * ones, which start first (the longest particles, the run's critical path).
+ * The reset that follows adds their count. */
+__global__ void handover_kernel(DsmcQueues Q, int p)
+{
+    const unsigned int n = Q.ctl[4], base = Q.ctl[2 * (1 - p)];
+    int32_t *dst = Q.fly[1 - p] + base;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (int64_t)n; i += stride) dst[i] = Q.svc[i];
+}
+
+/* add_svc: the service queue was handed over to the next fly queue. */
+__global__ void reset_kernel(unsigned int *ctl, int p, unsigned int *rounds, int add_svc)
 {
     if (threadIdx.x == 0) {
+        if (add_svc) ctl[2 * (1 - p)] += ctl[4];
         ctl[2 * p] = 0;
         ctl[2 * p + 1] = 0;
         ctl[4] = 0;
@@ -1090,25 +1101,23 @@ extern "C" hipError_t dsmc_launch_start(const DsmcEnv *E, int32_t run_id, DsmcRe
 }
 
 extern "C" hipError_t dsmc_launch_flight(const DsmcEnv *E, DsmcRec *rec, DsmcQueues Q, int p, int grid, int block,
-                                         int refill_batch, int64_t step_budget, int park_batch, int hop_to_fly,
-                                         hipStream_t st)
+                                         int refill_batch, int64_t step_budget, int park_batch, hipStream_t st)
 {
     if (!block_ok(block)) return hipErrorInvalidValue;
     const int mode = E->mode, adv = E->advanced;
     const size_t lds = sizeof(double) * (size_t)((E->ftd + 1) + (E->hd + 2) + 2 * (E->ffd + 1));
     if (mode == 1 && !adv)
         hipLaunchKernelGGL(lean_flight_kernel, dim3(grid), dim3(block), lds, st, *E, rec, Q, p, refill_batch,
-                           step_budget, park_batch > 0 ? park_batch : LEAN_PARK_BATCH, hop_to_fly);
-    else if (hop_to_fly)
-        return hipErrorInvalidValue;
+                           step_budget, park_batch > 0 ? park_batch : LEAN_PARK_BATCH);
     else
         DISPATCH(flight_kernel, dim3(grid), dim3(block), lds, st, *E, rec, Q, p, refill_batch, step_budget);
     return hipGetLastError();
 }
 
-extern "C" hipError_t dsmc_launch_reset(DsmcQueues Q, int p, hipStream_t st)
+extern "C" hipError_t dsmc_launch_handover(DsmcQueues Q, int p, int grid, hipStream_t st)
 {
-    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, Q.ctl, p, Q.ctl + 8);
+    hipLaunchKernelGGL(handover_kernel, dim3(grid), dim3(256), 0, st, Q, p);
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, Q.ctl, p, Q.ctl + 8, 1);
     return hipGetLastError();
 }
 
@@ -1147,7 +1156,7 @@ extern "C" hipError_t dsmc_launch_service(const DsmcEnv *E, DsmcRec *rec, DsmcQu
     else if (!adv) hipLaunchKernelGGL((service_kernel<1, 0, false>), dim3(grid), dim3(256), 0, st, *E,
                                       nullptr, rec, Q, p);
     else return hipErrorInvalidValue;
-    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, Q.ctl, p, Q.ctl + 8);
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, Q.ctl, p, Q.ctl + 8, 0);
     return hipGetLastError();
 }
 
@@ -1196,7 +1205,7 @@ extern "C" hipError_t dsmc_launch_phys(const DsmcEnv *E, const DsmcEnv *Ed, cons
         else PHYS_SINGLE(0, false);
 #undef PHYS_SINGLE
     }
-    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, Q.ctl, p, Q.ctl + 8);
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, Q.ctl, p, Q.ctl + 8, 0);
     return hipGetLastError();
 }
 
@@ -1206,7 +1215,7 @@ extern "C" hipError_t dsmc_launch_pool_service(const DsmcPoolDev *P, DsmcQueues 
     DsmcEnv none;
     memset(&none, 0, sizeof none);
     hipLaunchKernelGGL((service_kernel<1, 0, true>), dim3(grid), dim3(256), 0, st, none, P->envs, P->rec, Q, p);
-    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, Q.ctl, p, Q.ctl + 8);
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, Q.ctl, p, Q.ctl + 8, 0);
     return hipGetLastError();
 }
 
