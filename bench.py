@@ -433,7 +433,7 @@ def served_run(lib, args, shape, rate: float, seconds: float, drain: bool = True
     for i, sp in enumerate(specs):
         if sp.collisional:
             params[i], p2s[i] = W.coll_params(lib, sp.body, sp.n, sp.lon, sp.seed, hd=40)
-    sv = lib.dsmc_server_create(args.serve_budget_gb << 30, 256, 2)      # collisional requests accepted
+    sv = lib.dsmc_server_create(args.serve_budget_gb << 30, args.serve_max_inflight, 2)  # collisional accepted
     if not sv:
         raise RuntimeError("dsmc_server_create failed")
     bufs = [Buffers(p2s[i] if p2s[i] is not None else p, 5, per_particle=False, moments=False)
@@ -621,6 +621,7 @@ def main():
     ap.add_argument("--serve-sustained-seconds", type=float, default=10.0)
     ap.add_argument("--serve-scale", type=float, default=1.0)
     ap.add_argument("--serve-budget-gb", type=int, default=128)
+    ap.add_argument("--serve-max-inflight", type=int, default=256, help="requests the server holds at once")
     ap.add_argument("--serve-collisional", type=float, default=0.3, help="fraction of collisional requests")
     ap.add_argument("--no-coll", action="store_true")
     ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in run_ballistic files leg")

@@ -673,7 +673,13 @@ __device__ __attribute__((noinline)) CollideOut phys_collide(double vx, double v
     return o;
 }
 
-#define COLLIDE_BATCH 16
+/* Lanes waiting for a collision that start a collision round in the flight
+ * loop: single runs 48 (a round's latency, its loads of the cell's fit and the
+ * run's environment, is paid once for more lanes: a 2e5-particle Moon run 265
+ * against 343 ms of phase 2 with 16; Europa 302 against 377, Mercury 348
+ * against 407), pools 16 (the served mix: no difference). */
+#define COLLIDE_BATCH_SINGLE 48
+#define COLLIDE_BATCH_POOL 16
 
 __device__ __forceinline__ void lane_collide_lean(DsmcLane &L, const DsmcEnv *Ep)
 {
@@ -844,10 +850,11 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
                     if (L.phase == LANE_FLIGHT && L.steps >= budget_end) L.phase = LANE_IDLE + 8;
                 }
             }
-            /* collisions in a dense gas come every few steps: once COLLIDE_BATCH
+            /* collisions in a dense gas come every few steps: once COLLIDE_BATCH_*
              * lanes wait for one they take the lean path here, without ending
              * the flight loop; fewer wait for the service below */
-            if (MODE == 1 && ADV && __popcll(__ballot(L.phase == LANE_COLLIDE)) >= COLLIDE_BATCH &&
+            if (MODE == 1 && ADV &&
+                __popcll(__ballot(L.phase == LANE_COLLIDE)) >= (POOL ? COLLIDE_BATCH_POOL : COLLIDE_BATCH_SINGLE) &&
                 L.phase == LANE_COLLIDE)
                 lane_collide_lean(L, POOL ? P.envs + run : Ed);
             if constexpr (STAGE) stage_issue();
