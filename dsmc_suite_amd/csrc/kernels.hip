@@ -433,13 +433,6 @@ flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch, in
  * (a parked lane idles for the rest of its group; the vote costs every lane) */
 #define LEAN_STEPS 8
 #define LEAN_PARK_BATCH 4
-/* the landing-service kernels' physical ballistic loops take lean steps too */
-#ifndef LEAN_INLINE
-#define LEAN_INLINE 1
-#endif
-#ifndef LEAN_POOL
-#define LEAN_POOL 1
-#endif
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch, int64_t step_budget,
                    int pend_batch)
@@ -741,7 +734,7 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
      * lean steps' run constants (LeanEnv) and reads its run's full step
      * constants again for the cell logic: with all of them in registers the
      * lean steps spilled and the pool ran 22% slower */
-    constexpr bool LEAN = (!POOL || LEAN_POOL) && MODE == 1 && ADV == 0 && LEAN_INLINE;
+    constexpr bool LEAN = MODE == 1 && ADV == 0;
     DsmcLane L;
     L.phase = LANE_IDLE;
     DsmcStepEnv S = {};
