@@ -26,6 +26,7 @@ struct DsmcPoolDev {
     const double *hr;          /* radial threshold tables, [nhr] doubles in all */
     int ftd, ffd, hd, nhr;
     int mode, adv;
+    int steps_lds;             /* > 0: the pool kernels copy steps[0, steps_lds) into LDS */
 };
 
 extern "C" {

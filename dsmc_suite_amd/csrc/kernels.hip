@@ -700,7 +700,7 @@ __device__ __forceinline__ void lane_collide_lean(DsmcLane &L, const DsmcEnv *Ep
 /* Pools: up to 168 VGPRs (3 waves per SIMD), so that the lanes' per-run step
  * constants do not spill inside the flight loop (a reload waits behind the
  * wavefront's outstanding deposit atomics); single runs: 128 (4 waves). */
-template <bool POOL, int MODE, int ADV, bool STAGE>
+template <bool POOL, int MODE, int ADV, bool STAGE, bool SLDS = false>
 __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const DsmcEnv *Ed, const DsmcPoolDev &P,
                                                     DsmcRec *rec, const DsmcQueues &Q, int p, int svc_batch,
                                                     int64_t step_budget)
@@ -712,6 +712,17 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
     const double *gtq = POOL ? P.tq : E0.tq, *gcs = POOL ? P.cs : E0.cs, *ghr = POOL ? P.hr : E0.hr;
     for (int i = threadIdx.x; i < ntq + ncs + nhr; i += blockDim.x)
         lds_tabs[i] = i < ntq ? gtq[i] : (i < ntq + ncs ? gcs[i - ntq] : ghr[i - ntq - ncs]);
+    /* SLDS: the pool's run table behind the cell tables (the lanes' loads of
+     * their run's step constants for the cell logic then wait for no global
+     * access, such as the deposit atomics in flight) */
+    DsmcStepEnv *lds_steps = reinterpret_cast<DsmcStepEnv *>(lds_tabs + ntq + ncs + nhr);
+    if constexpr (SLDS) {
+        static_assert(sizeof(DsmcStepEnv) % 8 == 0, "the run table is copied in 8-byte words");
+        const unsigned long long *src = reinterpret_cast<const unsigned long long *>(P.steps);
+        unsigned long long *dst = reinterpret_cast<unsigned long long *>(lds_steps);
+        const int nw = P.steps_lds * (int)(sizeof(DsmcStepEnv) / 8);
+        for (int i = threadIdx.x; i < nw; i += blockDim.x) dst[i] = src[i];
+    }
     static_assert(!STAGE || MODE == 1, "staged deposits are physical");
     if constexpr (STAGE) stage_init();
     else if constexpr (!POOL) dep_init();
@@ -761,7 +772,7 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
                     slot = flyq[qi];
                     if constexpr (POOL && LEAN) {
                         run = rec[slot].run;
-                        TB.hr = lds_hr + P.steps[run].hr_off;
+                        TB.hr = lds_hr + (SLDS ? lds_steps[run].hr_off : P.steps[run].hr_off);
                     } else if (POOL) {
                         run = rec[slot].run;
                         S = P.steps[run];
@@ -771,7 +782,7 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
                     else rec_load_phys(L, rec[slot]);
                     budget_end = step_budget > 0 ? L.steps + step_budget : INT64_MAX;
                     if constexpr (LEAN && POOL) {
-                        const DsmcStepEnv Sr = P.steps[run];
+                        const DsmcStepEnv Sr = SLDS ? lds_steps[run] : P.steps[run];
                         const int64_t cap = Sr.max_steps > 0 ? Sr.max_steps : INT64_MAX;
                         N.stop = budget_end < cap ? budget_end : cap;
                         N.pend = false;
@@ -809,7 +820,7 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
                 if (N.pend) {
                     if constexpr (POOL) {
                         /* the run's step constants, read again for the cell logic */
-                        DsmcStepEnv Sr = P.steps[run];
+                        DsmcStepEnv Sr = SLDS ? lds_steps[run] : P.steps[run];
                         Sr.ftd = P.ftd; Sr.ffd = P.ffd; Sr.hd = P.hd;
                         if constexpr (STAGE) lean_tail<DsmcOpsStage>(L, Sr, TB, N);
                         else lean_tail<DsmcOpsDev>(L, Sr, TB, N);
@@ -877,7 +888,7 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
                 if (L.phase == LANE_FLIGHT) {       /* relaunched: a new hop from its new cell */
                     N.pend = false;
                     if constexpr (POOL) {
-                        DsmcStepEnv Sr = P.steps[run];
+                        DsmcStepEnv Sr = SLDS ? lds_steps[run] : P.steps[run];
                         Sr.ftd = P.ftd; Sr.ffd = P.ffd; Sr.hd = P.hd;
                         lean_refresh(N, L, Sr, TB);
                     } else {
@@ -915,12 +926,12 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
 /* STAGE: deposits to cell records staged and issued wavefront-wide (pools, and
  * single runs with velocity moments); single runs without moments keep the
  * LDS slot table for their count deposits. */
-template <bool POOL, int ADV, bool STAGE>
+template <bool POOL, int ADV, bool STAGE, bool SLDS = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(POOL ? 3 : 4)))
 phys_kernel(DsmcEnv E0, const DsmcEnv *Ed, DsmcPoolDev P, DsmcRec *rec, DsmcQueues Q, int p, int svc_batch,
             int64_t step_budget)
 {
-    inline_service_loop<POOL, 1, ADV, STAGE>(E0, Ed, P, rec, Q, p, svc_batch, step_budget);
+    inline_service_loop<POOL, 1, ADV, STAGE, SLDS>(E0, Ed, P, rec, Q, p, svc_batch, step_budget);
 }
 
 /* Compat ballistic particles (the reference's run_ballistic) with the landing
@@ -1182,6 +1193,10 @@ extern "C" hipError_t dsmc_launch_phys(const DsmcEnv *E, const DsmcEnv *Ed, cons
         memset(&none, 0, sizeof none);
         if (P->adv)
             hipLaunchKernelGGL((phys_kernel<true, 1, true>), dim3(grid), dim3(block), lds, st, none, (const DsmcEnv *)nullptr,
+                               *P, rec, Q, p, svc_batch, step_budget);
+        else if (P->steps_lds > 0)
+            hipLaunchKernelGGL((phys_kernel<true, 0, true, true>), dim3(grid), dim3(block),
+                               lds + (size_t)P->steps_lds * sizeof(DsmcStepEnv), st, none, (const DsmcEnv *)nullptr,
                                *P, rec, Q, p, svc_batch, step_budget);
         else
             hipLaunchKernelGGL((phys_kernel<true, 0, true>), dim3(grid), dim3(block), lds, st, none, (const DsmcEnv *)nullptr,

@@ -86,6 +86,8 @@ struct Pool {
 };
 
 enum { POOL_MAX_HR_TABS = 16, POOL_MAX_GEO = 4, POOL_MAX_HIST = 8, POOL_MAX_COLL = 8 };
+/* largest run table the pool kernels keep in LDS (else they read it from device memory) */
+#define POOL_STEPS_LDS_BYTES 16384
 
 /* Footprint in device memory of one particle slot and of one run's grid. */
 size_t pool_slot_bytes(int per_particle);

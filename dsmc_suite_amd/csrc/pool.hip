@@ -124,6 +124,8 @@ static DsmcPoolDev pool_dev(Pool *P)
     D.nhr = POOL_MAX_HR_TABS * (P->cls.hd + 2);
     D.mode = P->cls.mode;
     D.adv = P->cls.adv;
+    /* the run table in LDS when it is small (a 64-run batch: 12.8 KB per block) */
+    D.steps_lds = (size_t)P->max_runs * sizeof(DsmcStepEnv) <= POOL_STEPS_LDS_BYTES ? P->max_runs : 0;
     return D;
 }
 
