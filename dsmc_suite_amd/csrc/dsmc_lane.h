@@ -369,8 +369,22 @@ DSMC_HD int table_walk_pre(const double *tab, int k, int kmax, double v, const C
 template <class Ops, class EnvT>
 DSMC_HD void lane_flush(DsmcLane &L, const EnvT &E)
 {
-    if (L.acc_cell >= 0) Ops::deposit(E, L.acc_cell, L.acc_w, L.acx, L.acy, L.acz, L.acc_n);
+    if (L.acc_cell >= 0 && L.acc_n > 0) Ops::deposit(E, L.acc_cell, L.acc_w, L.acx, L.acy, L.acz, L.acc_n);
     L.acc_cell = -1;
+}
+
+/* A physical hop opens its first deposit visit at the launch cell with nothing
+ * in it: a first step that stays in that cell adds to it what a visit opened by
+ * that step would hold, one that leaves closes it with nothing deposited
+ * (lane_flush skips an empty visit), and the lean steps take the hop's first
+ * step too. */
+template <class EnvT>
+DSMC_HD void lane_open_visit(DsmcLane &L, const EnvT &E)
+{
+    if (L.ht < E.hd) {
+        L.acc_cell = (L.ht * E.ftd + L.td) * E.ffd + L.fd;
+        L.acc_w = 0.0; L.acx = 0.0; L.acy = 0.0; L.acz = 0.0; L.acc_n = 0;
+    }
 }
 
 /* WEIGHT: the visit's weight sum is kept (compat runs); physical particles all
@@ -712,16 +726,51 @@ DSMC_HD void lane_launch_compat(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
 }
 
 /* ------------------------------------------------------------------ physical emission */
+struct EmitDraws {
+    double u1, u2, u3, u4;
+};
+
+/* The four uniforms of an emission, words k .. k+7 of the particle's stream (the
+ * words lane_u01 would return four times in a row), from one out-of-line call
+ * that computes their two or three blocks together.  Drawn word by word, every
+ * draw was a call site of its own whose cached block the caller kept (spilled,
+ * in the register-heavy flight kernels) between draws. */
+#if defined(__HIP_DEVICE_COMPILE__)
+__attribute__((noinline))
+#endif
+DSMC_HD EmitDraws emit_draws(uint64_t seed, uint64_t pid, uint32_t stream, uint64_t k)
+{
+    const uint64_t b0 = k >> 2;
+    const uint32_t off = (uint32_t)(k & 3);
+    const dsmc_u32x4 a = dsmc_rng_block(seed, pid, stream, b0);
+    const dsmc_u32x4 b = dsmc_rng_block(seed, pid, stream, b0 + 1);
+    const dsmc_u32x4 c = off ? dsmc_rng_block(seed, pid, stream, b0 + 2) : b;
+    const uint32_t s[12] = {a.w[0], a.w[1], a.w[2], a.w[3], b.w[0], b.w[1], b.w[2], b.w[3],
+                            c.w[0], c.w[1], c.w[2], c.w[3]};
+    uint32_t w[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) w[i] = off == 0 ? s[i] : off == 1 ? s[i + 1] : off == 2 ? s[i + 2] : s[i + 3];
+    EmitDraws d;
+    d.u1 = dsmc_u01_from_words(w[0], w[1]);
+    d.u2 = dsmc_u01_from_words(w[2], w[3]);
+    d.u3 = dsmc_u01_from_words(w[4], w[5]);
+    d.u4 = dsmc_u01_from_words(w[6], w[7]);
+    return d;
+}
+
 DSMC_HD void lane_emit(DsmcLane &L, const DsmcEnv &E, double T)
 {
     double sig = sqrt(E.k_B * T / E.mass);
-    double u1 = lane_u01(L, E);
-    double u2 = lane_u01(L, E);
+    const EmitDraws D = emit_draws(E.seed, (uint64_t)L.pid, E.stream, L.k);
+    L.k += 8;
+    L.blk = ~0ull;                     /* the cached block is no longer the current one */
+    double u1 = D.u1;
+    double u2 = D.u2;
     double w = -dm_log(u1 * u2);
     double speed = sig * sqrt(2.0 * w);
-    double u3 = lane_u01(L, E);
+    double u3 = D.u3;
     double cos_t = sqrt(u3), sin_t = sqrt(1.0 - u3);
-    double phi = 2.0 * M_PI * lane_u01(L, E);
+    double phi = 2.0 * M_PI * D.u4;
     double sph, cph;
     dm_sincos(phi, &sph, &cph);
     double Ri = sqrt(L.sx * L.sx + L.sy * L.sy + L.sz * L.sz);
@@ -1005,6 +1054,7 @@ DSMC_HD void lane_start(DsmcLane &L, const DsmcEnv &E, int64_t pid, DsmcCounts &
     L.ncell = -1;
     L.hops = 1;
     lane_begin_flight(L, E);
+    lane_open_visit(L, E);
 }
 
 /* ------------------------------------------------------------------ service */
@@ -1048,6 +1098,7 @@ DSMC_HD void lane_service(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
     lane_emit(L, E, Tf);
     L.hops++;
     lane_begin_flight(L, E, true);
+    lane_open_visit(L, E);
 }
 
 /* ------------------------------------------------------------------ collision (advanced) */
