@@ -1263,21 +1263,41 @@ DSMC_HD LeanEnv lean_env(const EnvT &E)
     return F;
 }
 
+/* The lean bounds of cell (td, fd, ht): a function of the cell alone. */
+template <class EnvT>
+DSMC_HD CellPre lean_bounds(int td, int fd, int ht, const EnvT &E, const DsmcCellTabs &TB)
+{
+    const CellPre P = cell_pre(td, fd, ht, TB.tq, TB.cs, TB.hr);
+    CellPre B;
+    B.tlo = P.tlo + LEAN_TQ_SHIFT;
+    B.thi = P.thi - LEAN_TQ_SHIFT;
+    B.c0 = P.c0 - P.s0 * LEAN_CS_SHIFT;
+    B.s0 = P.s0 + P.c0 * LEAN_CS_SHIFT;
+    B.c1 = P.c1 + P.s1 * LEAN_CS_SHIFT;
+    B.s1 = P.s1 - P.c1 * LEAN_CS_SHIFT;
+    const double r_up = nextafter(E.R, INFINITY);          /* Rn > R  <=>  Rn >= r_up */
+    B.rlo = P.rlo > r_up ? P.rlo : r_up;
+    B.rhi = P.rhi < E.max_height ? P.rhi : E.max_height;
+    return B;
+}
+
+/* The shortcut flag and the sunlit limit after the cell logic ran or a hop began
+ * (the bounds: lean_bounds). */
+template <class EnvT>
+DSMC_HD void lean_refresh_flags(LeanLane &N, const DsmcLane &L, const EnvT &E)
+{
+    N.easy = L.ht >= E.hd || L.acc_cell == (L.ht * E.ftd + L.td) * E.ffd + L.fd;
+    const int64_t lim = L.n_photo - L.n_day - 1;
+    N.sun_lim = lim < (int64_t)INT32_MAX ? (int32_t)lim : INT32_MAX;
+    N.sun_steps = 0;
+}
+
 /* Refreshes the lean bounds, the shortcut flag and the sunlit limit after the
  * cell logic ran. */
 template <class EnvT>
 DSMC_HD void lean_refresh(LeanLane &N, const DsmcLane &L, const EnvT &E, const DsmcCellTabs &TB)
 {
-    const CellPre P = cell_pre(L.td, L.fd, L.ht, TB.tq, TB.cs, TB.hr);
-    N.B.tlo = P.tlo + LEAN_TQ_SHIFT;
-    N.B.thi = P.thi - LEAN_TQ_SHIFT;
-    N.B.c0 = P.c0 - P.s0 * LEAN_CS_SHIFT;
-    N.B.s0 = P.s0 + P.c0 * LEAN_CS_SHIFT;
-    N.B.c1 = P.c1 + P.s1 * LEAN_CS_SHIFT;
-    N.B.s1 = P.s1 - P.c1 * LEAN_CS_SHIFT;
-    const double r_up = nextafter(E.R, INFINITY);          /* Rn > R  <=>  Rn >= r_up */
-    N.B.rlo = P.rlo > r_up ? P.rlo : r_up;
-    N.B.rhi = P.rhi < E.max_height ? P.rhi : E.max_height;
+    N.B = lean_bounds(L.td, L.fd, L.ht, E, TB);
     N.easy = L.ht >= E.hd || L.acc_cell == (L.ht * E.ftd + L.td) * E.ffd + L.fd;
     const int64_t lim = L.n_photo - L.n_day - 1;
     N.sun_lim = lim < (int64_t)INT32_MAX ? (int32_t)lim : INT32_MAX;

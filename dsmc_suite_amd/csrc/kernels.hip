@@ -786,7 +786,6 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
                         const int64_t cap = Sr.max_steps > 0 ? Sr.max_steps : INT64_MAX;
                         N.stop = budget_end < cap ? budget_end : cap;
                         N.pend = false;
-                        FE = lean_env(Sr);
                         lean_refresh(N, L, Sr, TB);
                     } else if constexpr (LEAN) {
                         const int64_t cap = E0.max_steps > 0 ? E0.max_steps : INT64_MAX;
@@ -803,6 +802,9 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
             continue;
         }
         if (POOL && !LEAN) { S.ftd = P.ftd; S.ffd = P.ffd; S.hd = P.hd; }
+        /* pools: the lean steps' run constants, read again (LDS) for each flight
+         * phase rather than held through the service */
+        if constexpr (POOL && LEAN) FE = lean_env(SLDS ? lds_steps[run] : P.steps[run]);
         for (;;) {
           if constexpr (LEAN) {
             const bool fl = L.phase == LANE_FLIGHT;
@@ -890,12 +892,21 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
                     if constexpr (POOL) {
                         DsmcStepEnv Sr = SLDS ? lds_steps[run] : P.steps[run];
                         Sr.ftd = P.ftd; Sr.ffd = P.ffd; Sr.hd = P.hd;
-                        lean_refresh(N, L, Sr, TB);
+                        lean_refresh_flags(N, L, Sr);
                     } else {
                         lean_refresh(N, L, E0, TB);
                     }
                 }
             }
+        }
+        /* pools: every lane's lean bounds are computed again from its cell (LDS
+         * tables) after the service round rather than held through it, so that
+         * the service has their registers; a lane without a particle in flight
+         * computes them for cell 0 and never reads them */
+        if constexpr (POOL && LEAN) {
+            const bool fl = L.phase == LANE_FLIGHT;
+            const DsmcStepEnv &Sr = SLDS ? lds_steps[run] : P.steps[run];
+            N.B = lean_bounds(fl ? L.td : 0, fl ? L.fd : 0, fl ? L.ht : 0, Sr, TB);
         }
         if constexpr (STAGE) stage_issue();
         const bool fin = svc && L.phase == LANE_DONE;
