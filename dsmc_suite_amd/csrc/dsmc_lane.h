@@ -77,6 +77,10 @@ struct DsmcEnv {
     const double *fnorm;      /* finest level, [nfine*9]: the same normals, unit length */
     const int32_t *fmap;      /* [6*G*G*DSMC_FMAP_K] candidate finest faces per cube-map cell, -1 = none */
     int fmap_g;               /* G; fmap == nullptr: no map */
+    /* [6*G*G][4][9]: fnorm's rows of each cell's first four candidates (a -1
+     * candidate: the first one's), so the first test needs no face index;
+     * nullptr: the candidates' rows are read from fnorm */
+    const double *fmap_n;
     int level_off[12];
     /* advanced inputs */
     const double *copy_atm;   /* scaled phase-1 density, [atm_total] */
@@ -836,6 +840,15 @@ DSMC_HD int locate_face(const DsmcEnv &E, double dx, double dy, double dz)
     return best;
 }
 
+/* Direction d at least eps inside the face of unit edge normals u[0..8]. */
+DSMC_HD bool normals_hold(const double *u, double dx, double dy, double dz, double eps)
+{
+    const bool a = dx * u[0] + dy * u[1] + dz * u[2] >= eps;
+    const bool b = dx * u[3] + dy * u[4] + dz * u[5] >= eps;
+    const bool c = dx * u[6] + dy * u[7] + dz * u[8] >= eps;
+    return a & b & c;
+}
+
 /* Direction d at least eps inside finest face f (all three unit edge normals). */
 DSMC_HD bool face_holds(const DsmcEnv &E, int f, double dx, double dy, double dz, double eps)
 {
@@ -884,8 +897,23 @@ DSMC_HD int locate_face_near(const DsmcEnv &E, int hint, double dx, double dy, d
     const int cell = E.fmap ? fmap_cell(E.fmap_g, dx, dy, dz) : -1;
     if (cell >= 0) {
         const FmapQuad *q = (const FmapQuad *)(E.fmap + (size_t)cell * DSMC_FMAP_K);
+        int g0 = 0;
+        if (E.fmap_n) {
+            /* the first four candidates' normals stored with the cell: loaded
+             * together with the candidates, one memory round trip */
+            const FmapQuad c = q[0];
+            const double *nn = E.fmap_n + (size_t)cell * 36;
+            bool in[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) in[j] = normals_hold(nn + 9 * j, dx, dy, dz, eps);
+            if (c.f[0] < 0) return locate_face(E, dx, dy, dz);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (in[j] && c.f[j] >= 0) return c.f[j];
+            g0 = 1;
+        }
 #pragma unroll 1
-        for (int g = 0; g < DSMC_FMAP_K / 4; g++) {
+        for (int g = g0; g < DSMC_FMAP_K / 4; g++) {
             const FmapQuad c = q[g];
             if (c.f[0] < 0) break;
             bool in[4];

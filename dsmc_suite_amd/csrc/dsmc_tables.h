@@ -297,6 +297,24 @@ inline void build_face_map(const DsmcEnv &E, const HostTables &T, int G, std::ve
     }
 }
 
+/* fnorm's rows of the first four candidates of every face-map cell (a -1
+ * candidate takes the first candidate's row; a cell without candidates, zeros),
+ * [cells][4][9]: the layout of DsmcEnv::fmap_n. */
+inline void build_face_map_normals(const std::vector<int32_t> &fmap, const std::vector<double> &fnorm, int G,
+                                   std::vector<double> &out)
+{
+    const size_t cells = (size_t)6 * G * G;
+    out.assign(cells * 36, 0.0);
+    for (size_t c = 0; c < cells; c++) {
+        const int32_t *f = &fmap[c * DSMC_FMAP_K];
+        if (f[0] < 0) continue;
+        for (int j = 0; j < 4; j++) {
+            const int32_t face = f[j] >= 0 ? f[j] : f[0];
+            for (int k = 0; k < 9; k++) out[c * 36 + 9 * j + k] = fnorm[(size_t)face * 9 + k];
+        }
+    }
+}
+
 }  // namespace dsmc
 
 #endif /* DSMC_TABLES_H */

@@ -57,7 +57,7 @@ struct Pool {
     DevBuf coll_cf, coll_lo, coll_hi, coll_nsig, coll_valid, coll_cnt;
     std::vector<int> hist_used, coll_used;
     DevBuf envs_d, steps_d, tq, cs, hr;
-    DevBuf geo_vert, geo_faces, geo_enorm, geo_fnorm, geo_fmap;   /* [max_geo] slots */
+    DevBuf geo_vert, geo_faces, geo_enorm, geo_fnorm, geo_fmap, geo_fmapn;   /* [max_geo] slots */
     int fmap_g = 0;                            /* face-map resolution of the class */
     DevBuf cdf, face_T;                        /* [max_runs][nfine] */
     DevBuf vcount, cellacc, horizon, counters; /* [max_runs] grid slots (cellacc: 32-byte cell records) */

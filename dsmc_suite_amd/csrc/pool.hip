@@ -227,6 +227,7 @@ int pool_create(Pool *P, const PoolClass &c, int max_runs, int64_t max_slots, in
     P->fmap_g = face_map_grid((int)P->nfine);
     const size_t fmap_n = (size_t)6 * P->fmap_g * P->fmap_g * DSMC_FMAP_K;
     if ((rc = P->geo_fmap.alloc(POOL_MAX_GEO * fmap_n * 4))) return rc;
+    if ((rc = P->geo_fmapn.alloc(POOL_MAX_GEO * fmap_n / DSMC_FMAP_K * 36 * 8))) return rc;
     if ((rc = P->cdf.alloc(mr * P->nfine * 8)) || (rc = P->face_T.alloc(mr * P->nfine * 8))) return rc;
     if ((rc = P->vcount.alloc(mr * P->cells * 8)) || (rc = P->cellacc.alloc(mr * P->cells * 32))) return rc;
     if ((rc = P->horizon.alloc(mr * c.ftd * c.ffd * 8)) || (rc = P->counters.zeros(mr * 64, st))) return rc;
@@ -434,6 +435,7 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out, int hist, 
     double *fnorm_d = (double *)P->geo_fnorm.ptr + (size_t)geo * P->nfine * 9;
     const size_t fmap_n = (size_t)6 * P->fmap_g * P->fmap_g * DSMC_FMAP_K;
     int32_t *fmap_d = (int32_t *)P->geo_fmap.ptr + (size_t)geo * fmap_n;
+    double *fmapn_d = (double *)P->geo_fmapn.ptr + (size_t)geo * (fmap_n / DSMC_FMAP_K) * 36;
     if (new_geo) {
         /* a slot no queued work reads (its last user has finished); synchronous
          * copies from the caller's pageable arrays, once per geometry */
@@ -444,6 +446,9 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out, int hist, 
         std::vector<int32_t> fm;
         build_face_map(pr.E, T, P->fmap_g, fm);
         HIP_OK(hipMemcpy(fmap_d, fm.data(), fmap_n * 4, hipMemcpyHostToDevice));
+        std::vector<double> fmn;
+        build_face_map_normals(fm, T.fnorm, P->fmap_g, fmn);
+        HIP_OK(hipMemcpy(fmapn_d, fmn.data(), fmn.size() * 8, hipMemcpyHostToDevice));
         P->geos[geo].key = key;
         P->geos[geo].valid = 1;
     }
@@ -471,6 +476,7 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out, int hist, 
     E.fnorm = fnorm_d;
     E.fmap = fmap_d;
     E.fmap_g = P->fmap_g;
+    E.fmap_n = fmapn_d;
     E.density = nullptr;
     E.density_from_count = 1;
     E.vcount = (unsigned long long *)P->vcount.ptr + (size_t)g * cells;
