@@ -113,7 +113,7 @@ static int coordinate_descent(dsmc_launch_shape *best, Trial trial, double *best
         if ((rc = try_shape(c))) return rc;
     }
     base = *best;
-    for (int r : {16, 32, 48, 64}) {
+    for (int r : {16, 32, 40, 48, 64}) {
         if (r == base.relaunch_batch) continue;
         dsmc_launch_shape c = base;
         c.relaunch_batch = r;

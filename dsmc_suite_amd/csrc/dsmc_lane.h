@@ -889,6 +889,10 @@ DSMC_HD int fmap_cell(int G, double dx, double dy, double dz)
  * when none holds d that firmly, or |d| is zero or not finite, the search runs. */
 struct alignas(16) FmapQuad { int32_t f[4]; };
 
+/* FMN: the caller's environment may hold fmap_n (pools); without it the code
+ * for the stored normals is not compiled in (the single-run kernels, whose
+ * register budget it would tighten). */
+template <bool FMN = false>
 DSMC_HD int locate_face_near(const DsmcEnv &E, int hint, double dx, double dy, double dz, double dlen)
 {
     if (!E.fnorm || !(dlen > 0.0 && dlen <= 1.0e308)) return locate_face(E, dx, dy, dz);
@@ -898,7 +902,7 @@ DSMC_HD int locate_face_near(const DsmcEnv &E, int hint, double dx, double dy, d
     if (cell >= 0) {
         const FmapQuad *q = (const FmapQuad *)(E.fmap + (size_t)cell * DSMC_FMAP_K);
         int g0 = 0;
-        if (E.fmap_n) {
+        if (FMN && E.fmap_n) {
             /* the first four candidates' normals stored with the cell: loaded
              * together with the candidates, one memory round trip */
             const FmapQuad c = q[0];
@@ -1087,7 +1091,7 @@ DSMC_HD void lane_start(DsmcLane &L, const DsmcEnv &E, int64_t pid, DsmcCounts &
 
 /* ------------------------------------------------------------------ service */
 /* After a flight ends on the surface (or the per-hop step limit). */
-template <class Ops, int MODE, int ADV>
+template <class Ops, int MODE, int ADV, bool FMN = false>
 DSMC_HD void lane_service(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
 {
     lane_flush<Ops>(L, E);
@@ -1112,7 +1116,7 @@ DSMC_HD void lane_service(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
      * landings of the 64-run batch and the face map's first four candidates in
      * ~84%, so testing the launch face first costs most landings a dependent
      * round trip to memory (the 64-run batch: 2% faster without it) */
-    int f = locate_face_near(E, -1, L.sx, L.sy, L.sz, Rl);
+    int f = locate_face_near<FMN>(E, -1, L.sx, L.sy, L.sz, Rl);
     double Tf = E.face_T[f];
     if (Tf < E.t_freeze) {
         /* the horizon cell of sph_index(pos_angles(...)) by the exact walks from

@@ -432,6 +432,10 @@ flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch, in
  * 6 / 8 / 12 / 16 take 157 / 145 / 139 / 137 / 141 / 145 ms of flight kernel
  * (a parked lane idles for the rest of its group; the vote costs every lane) */
 #define LEAN_STEPS 8
+/* pools: their short hops park lanes every ~18 steps, so a vote costs more per
+ * step (the 64-run batch: 4 / 6 / 8 / 10 / 12 / 16 / 20 lean steps between
+ * votes, 1056 / 978 / 945 / 929 / 922 / 928 / 946 ms) */
+#define LEAN_STEPS_POOL 12
 #define LEAN_PARK_BATCH 4
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 lean_flight_kernel(DsmcEnv E, DsmcRec *rec, DsmcQueues Q, int p, int refill_batch, int64_t step_budget,
@@ -617,7 +621,7 @@ pool_flight_kernel(DsmcPoolDev P, DsmcQueues Q, int p, int refill_batch, int64_t
  * load per use).  phys_service is the out-of-line form (single runs: the flight
  * kernel's registers stay those of the Verlet step; the lane state travels
  * through the call frame); pools inline the body. */
-template <class Ops, int ADV>
+template <class Ops, int ADV, bool FMN = false>
 __device__ __forceinline__ void phys_service_body(DsmcLane &T, const DsmcEnv *Ep, DsmcCounts &cnt,
                                                   const DsmcCellTabs TB)
 {
@@ -626,7 +630,7 @@ __device__ __forceinline__ void phys_service_body(DsmcLane &T, const DsmcEnv *Ep
     E.tq = TB.tq;
     E.cs = TB.cs;
     E.hr = TB.hr;
-    lane_service<Ops, 1, ADV>(T, E, cnt);
+    lane_service<Ops, 1, ADV, FMN>(T, E, cnt);
 }
 
 template <class Ops, int ADV>
@@ -810,7 +814,7 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
             const bool fl = L.phase == LANE_FLIGHT;
             bool act = fl && !N.pend;
 #pragma unroll
-            for (int u = 0; u < LEAN_STEPS; u++) {
+            for (int u = 0; u < (POOL ? LEAN_STEPS_POOL : LEAN_STEPS); u++) {
                 if (act) {
                     if constexpr (POOL) act = lane_fast_phys(L, FE, N);
                     else act = lane_fast_phys(L, E0, N);
@@ -881,8 +885,8 @@ __device__ __forceinline__ void inline_service_loop(const DsmcEnv &E0, const Dsm
             /* pools: the service inline (the lane state needs no call frame; the
              * 64-run batch 8.96e10 against 8.62e10 particle-steps/s); single runs
              * out of line (inline, the 1e6-particle Ceres run took 38 against 30 ms) */
-            else if constexpr (POOL && STAGE) phys_service_body<DsmcOpsStage, ADV>(T, P.envs + run, cnt, TB);
-            else if constexpr (POOL) phys_service_body<DsmcOpsDev, ADV>(T, P.envs + run, cnt, TB);
+            else if constexpr (POOL && STAGE) phys_service_body<DsmcOpsStage, ADV, true>(T, P.envs + run, cnt, TB);
+            else if constexpr (POOL) phys_service_body<DsmcOpsDev, ADV, true>(T, P.envs + run, cnt, TB);
             else if constexpr (STAGE) phys_service<DsmcOpsStage, ADV>(T, Ed, cnt, TB);
             else phys_service<DsmcOpsLds, ADV>(T, Ed, cnt, TB);
             L = T;
