@@ -167,7 +167,15 @@ static int tune_pool(Pool *P, const std::vector<PoolPrep> &preps, double frac, d
         *score = (double)steps / ms;
         return DSMC_OK;
     };
+    /* the descent starts from the shape that ran the 64-run batch fastest (three
+     * 256-lane blocks per CU, services of 40 lanes, 4096-step rounds): it keeps
+     * a coordinate's current value unless another beats it by 1% on the sample */
     dsmc_launch_shape best = P->shape;
+    best.block = 256;
+    best.blocks_per_cu = 3;
+    best.relaunch_batch = 40;
+    best.step_budget = 4096;
+    best.service_inline = 1;
     double score = 0.0;
     int rc = coordinate_descent(&best, trial, &score);
     if (rc) return rc;
