@@ -71,7 +71,7 @@ struct TimedStream {
 struct RunDev {
     DsmcEnv E;
     HostTables T;
-    DevBuf tq, hr, cs, vt_speeds, vt_count, Tsurf, cdf, enorm, fnorm, fmap, vert, faces, face_T;
+    DevBuf tq, hr, cs, vt_speeds, vt_count, Tsurf, cdf, enorm, fnorm, fmap, vert, faces, face_T, face_sig;
     DevBuf density, horizon, vmom, vcount, cellacc, ttime, end_state, steps, fate, hops;
     DevBuf vrec, vrec_i, count_nn, ctl;
     /* advanced */

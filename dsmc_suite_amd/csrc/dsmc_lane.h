@@ -71,6 +71,7 @@ struct DsmcEnv {
     int levels, nfine;
     const double *cdf;        /* [nfine] inclusive prefix sum of launch weights */
     const double *face_T;     /* [nfine] */
+    const double *face_sig;   /* [nfine] sqrt(k_B face_T / mass): the emission's thermal speed at each face */
     const double *vert;       /* [nvert*3] */
     const int32_t *faces;     /* all levels, [sum*3] */
     const double *enorm;      /* all levels, [sum*9]: cross(a,b), cross(b,c), cross(c,a) */
@@ -762,9 +763,9 @@ DSMC_HD EmitDraws emit_draws(uint64_t seed, uint64_t pid, uint32_t stream, uint6
     return d;
 }
 
-DSMC_HD void lane_emit(DsmcLane &L, const DsmcEnv &E, double T)
+/* sig: sqrt(k_B T / m) at the launch face (DsmcEnv::face_sig). */
+DSMC_HD void lane_emit(DsmcLane &L, const DsmcEnv &E, double sig)
 {
-    double sig = sqrt(E.k_B * T / E.mass);
     const EmitDraws D = emit_draws(E.seed, (uint64_t)L.pid, E.stream, L.k);
     L.k += 8;
     L.blk = ~0ull;                     /* the cached block is no longer the current one */
@@ -1078,7 +1079,7 @@ DSMC_HD void lane_start(DsmcLane &L, const DsmcEnv &E, int64_t pid, DsmcCounts &
     double nph = ceil(Ex * E.lifetime / E.h);
     L.n_photo = nph > 9.0e18 ? (int64_t)9000000000000000000LL : (int64_t)nph;
     L.face = face;
-    lane_emit(L, E, E.face_T[face]);
+    lane_emit(L, E, E.face_sig[face]);
     if (ADV) {                       /* collision depth threshold ~ Exp(1) */
         L.real_col_p = 0.0;
         L.col_p = -dm_log(lane_u01(L, E));
@@ -1133,7 +1134,7 @@ DSMC_HD void lane_service(DsmcLane &L, const DsmcEnv &E, DsmcCounts &cnt)
     double sc = E.R / Rl;
     L.sx *= sc; L.sy *= sc; L.sz *= sc;
     L.face = f;
-    lane_emit(L, E, Tf);
+    lane_emit(L, E, E.face_sig[f]);
     L.hops++;
     lane_begin_flight(L, E, true);
     lane_open_visit(L, E);

@@ -80,6 +80,7 @@ struct HostTables {
     std::vector<double> tq, hr, cs, vt_speeds;
     std::vector<int32_t> vt_count;
     std::vector<double> cdf, enorm, fnorm;
+    std::vector<double> face_sig;   /* sqrt(k_B T / mass) per finest face */
     std::vector<int32_t> fmap;      /* build_face_map, where a surface is uploaded */
     std::vector<int32_t> fmap_lat;  /* build_face_map's lattice (kept, so a reused HostTables reallocates nothing) */
     int level_off[12];
@@ -201,6 +202,10 @@ inline int build_env(const dsmc_params *p, const dsmc_surface *s, int advanced, 
             T->cdf[i] = acc;
         }
         if (!(acc > 0)) return DSMC_ERR_ARG;
+        /* the emission's thermal speed at each face, as the oracle computes it per
+         * emission (correctly rounded division and square root on both sides) */
+        T->face_sig.assign(T->nfine, 0.0);
+        for (int i = 0; i < T->nfine; i++) T->face_sig[i] = sqrt(E->k_B * s->face_T[i] / E->mass);
         T->enorm.assign((size_t)off * 9, 0.0);
         for (int f = 0; f < off; f++) {
             const int32_t *fv = s->faces + 3 * (size_t)f;

@@ -146,6 +146,7 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
         UP(rd->cdf, T.cdf);
         UP(rd->enorm, T.enorm);
         UP(rd->fnorm, T.fnorm);
+        UP(rd->face_sig, T.face_sig);
         E.fmap_g = face_map_grid(T.nfine);
         build_face_map(E, T, E.fmap_g, T.fmap, &T.fmap_lat);
         UP(rd->fmap, T.fmap);
@@ -159,6 +160,7 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
         E.vert = (const double *)rd->vert.ptr;
         E.faces = (const int32_t *)rd->faces.ptr;
         E.face_T = (const double *)rd->face_T.ptr;
+        E.face_sig = (const double *)rd->face_sig.ptr;
         /* short hops: the body holds its gas close (escape speed more than twice
          * the mean thermal speed at the hottest face), so a particle lands every
          * few dozen steps and the landing service belongs in the flight kernel */
@@ -255,7 +257,7 @@ int stage_run(const dsmc_params *p, const dsmc_surface *s, const dsmc_result *r,
 void RunDev::drop_buffers()
 {
     DevBuf *all[] = {&tq, &hr, &cs, &vt_speeds, &vt_count, &Tsurf, &cdf, &enorm, &fnorm, &fmap, &vert, &faces,
-                     &face_T, &density, &horizon, &vmom, &vcount, &cellacc, &ttime, &end_state, &steps, &fate,
+                     &face_T, &face_sig, &density, &horizon, &vmom, &vcount, &cellacc, &ttime, &end_state, &steps, &fate,
                      &hops, &vrec, &vrec_i, &count_nn, &ctl, &atm, &count_num, &coeffs, &pmin, &pmax, &pvalid,
                      &speeds, &nsig, &vhist, &rec, &flyq0, &flyq1, &svcq, &qctl, &counters, &rec_total, &env_d};
     for (DevBuf *b : all) b->release();

@@ -59,13 +59,13 @@ struct Pool {
     DevBuf envs_d, steps_d, tq, cs, hr;
     DevBuf geo_vert, geo_faces, geo_enorm, geo_fnorm, geo_fmap, geo_fmapn;   /* [max_geo] slots */
     int fmap_g = 0;                            /* face-map resolution of the class */
-    DevBuf cdf, face_T;                        /* [max_runs][nfine] */
+    DevBuf cdf, face_T, face_sig;              /* [max_runs][nfine] */
     DevBuf vcount, cellacc, horizon, counters; /* [max_runs] grid slots (cellacc: 32-byte cell records) */
     DevBuf end_state, steps_o, fate, hops;     /* [max_slots], optional */
 
     DsmcEnv *envs_h = nullptr;                 /* pinned host copies */
     DsmcStepEnv *steps_h = nullptr;
-    double *stage_h = nullptr;                 /* [max_runs][2][nfine] */
+    double *stage_h = nullptr;                 /* [max_runs][3][nfine] */
     unsigned long long *cnt_h = nullptr;       /* [max_runs*8] */
     unsigned int *ctl_h = nullptr;             /* [16] */
     void *out_h = nullptr;                     /* result staging */

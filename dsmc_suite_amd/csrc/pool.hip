@@ -228,7 +228,9 @@ int pool_create(Pool *P, const PoolClass &c, int max_runs, int64_t max_slots, in
     const size_t fmap_n = (size_t)6 * P->fmap_g * P->fmap_g * DSMC_FMAP_K;
     if ((rc = P->geo_fmap.alloc(POOL_MAX_GEO * fmap_n * 4))) return rc;
     if ((rc = P->geo_fmapn.alloc(POOL_MAX_GEO * fmap_n / DSMC_FMAP_K * 36 * 8))) return rc;
-    if ((rc = P->cdf.alloc(mr * P->nfine * 8)) || (rc = P->face_T.alloc(mr * P->nfine * 8))) return rc;
+    if ((rc = P->cdf.alloc(mr * P->nfine * 8)) || (rc = P->face_T.alloc(mr * P->nfine * 8)) ||
+        (rc = P->face_sig.alloc(mr * P->nfine * 8)))
+        return rc;
     if ((rc = P->vcount.alloc(mr * P->cells * 8)) || (rc = P->cellacc.alloc(mr * P->cells * 32))) return rc;
     if ((rc = P->horizon.alloc(mr * c.ftd * c.ffd * 8)) || (rc = P->counters.zeros(mr * 64, st))) return rc;
     if (per_particle) {
@@ -258,7 +260,7 @@ int pool_create(Pool *P, const PoolClass &c, int max_runs, int64_t max_slots, in
     }
     if (hipHostMalloc((void **)&P->envs_h, mr * sizeof(DsmcEnv), 0) != hipSuccess ||
         hipHostMalloc((void **)&P->steps_h, mr * sizeof(DsmcStepEnv), 0) != hipSuccess ||
-        hipHostMalloc((void **)&P->stage_h, mr * 2 * P->nfine * 8, 0) != hipSuccess ||
+        hipHostMalloc((void **)&P->stage_h, mr * 3 * P->nfine * 8, 0) != hipSuccess ||
         hipHostMalloc((void **)&P->cnt_h, POOL_RING * mr * 64, 0) != hipSuccess ||
         hipHostMalloc((void **)&P->ctl_h, POOL_RING * 16 * sizeof(unsigned int), 0) != hipSuccess)
         return DSMC_ERR_NOMEM;
@@ -456,13 +458,16 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out, int hist, 
     P->geos[geo].ref++;
 
     const size_t nf = P->nfine, cells = P->cells;
-    double *stage = P->stage_h + (size_t)g * 2 * nf;
+    double *stage = P->stage_h + (size_t)g * 3 * nf;
     memcpy(stage, T.cdf.data(), nf * 8);
     memcpy(stage + nf, pr.face_T.data(), nf * 8);
+    memcpy(stage + 2 * nf, T.face_sig.data(), nf * 8);
     double *cdf_d = (double *)P->cdf.ptr + (size_t)g * nf;
     double *fT_d = (double *)P->face_T.ptr + (size_t)g * nf;
+    double *fs_d = (double *)P->face_sig.ptr + (size_t)g * nf;
     HIP_OK(hipMemcpyAsync(cdf_d, stage, nf * 8, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(fT_d, stage + nf, nf * 8, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(fs_d, stage + 2 * nf, nf * 8, hipMemcpyHostToDevice, st));
 
     DsmcEnv &E = R.E;
     E.tq = (const double *)P->tq.ptr;
@@ -470,6 +475,7 @@ int pool_admit(Pool *P, const PoolPrep &pr, uint64_t tag, int *g_out, int hist, 
     E.hr = hr_d;
     E.cdf = cdf_d;
     E.face_T = fT_d;
+    E.face_sig = fs_d;
     E.vert = vert_d;
     E.faces = faces_d;
     E.enorm = enorm_d;

@@ -293,7 +293,7 @@ extern "C" int emu_ballistic_path(const dsmc_params *p, const dsmc_surface *s, d
     E.T = s->T_latlon;
     E.vt_speeds = T.vt_speeds.data(); E.vt_count = T.vt_count.data();
     E.tq = T.tq.data(); E.hr = T.hr.data(); E.cs = T.cs.data();
-    E.cdf = T.cdf.data(); E.face_T = s->face_T; E.vert = s->vert; E.faces = s->faces;
+    E.cdf = T.cdf.data(); E.face_T = s->face_T; E.face_sig = T.face_sig.data(); E.vert = s->vert; E.faces = s->faces;
     E.enorm = T.enorm.data();
     E.fnorm = T.fnorm.data();
     attach_face_map(E, T);
@@ -363,7 +363,7 @@ extern "C" int emu_physical_coll(const dsmc_params *p, const dsmc_surface *s, co
     std::vector<double> den(cells, 0.0), hor((size_t)p->ftd * p->ffd, 0.0);
     std::vector<unsigned long long> vcount(cells, 0);
     E.tq = T.tq.data(); E.hr = T.hr.data(); E.cs = T.cs.data();
-    E.cdf = T.cdf.data(); E.face_T = s->face_T; E.vert = s->vert; E.faces = s->faces;
+    E.cdf = T.cdf.data(); E.face_T = s->face_T; E.face_sig = T.face_sig.data(); E.vert = s->vert; E.faces = s->faces;
     E.enorm = T.enorm.data(); E.fnorm = T.fnorm.data();
     attach_face_map(E, T);
     E.density = den.data(); E.horizon = hor.data();
