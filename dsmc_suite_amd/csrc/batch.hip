@@ -93,7 +93,7 @@ static int coordinate_descent(dsmc_launch_shape *best, Trial trial, double *best
         double sc = -1.0;
         int r = trial(c, &sc);
         if (r) return r;
-        if (sc > bs * 1.01) { bs = sc; *best = c; }
+        if (sc > bs * 1.02) { bs = sc; *best = c; }
         return DSMC_OK;
     };
     const int waves_per_cu[] = {8, 12, 16, 20, 24};
@@ -169,7 +169,7 @@ static int tune_pool(Pool *P, const std::vector<PoolPrep> &preps, double frac, d
     };
     /* the descent starts from the shape that ran the 64-run batch fastest (three
      * 256-lane blocks per CU, services of 40 lanes, 4096-step rounds): it keeps
-     * a coordinate's current value unless another beats it by 1% on the sample */
+     * a coordinate's current value unless another beats it by 2% on the sample */
     dsmc_launch_shape best = P->shape;
     best.block = 256;
     best.blocks_per_cu = 3;
@@ -228,6 +228,7 @@ extern "C" dsmc_batch *dsmc_batch_create(int nruns, const dsmc_params *params, c
     int rc = (flags & ~(DSMC_BATCH_PER_PARTICLE | DSMC_BATCH_DENSITY_ONLY)) ? DSMC_ERR_ARG : DSMC_OK;
     if (!rc) rc = batch_prepare(nruns, params, surfaces, &b->preps, &cls, &total);
     for (auto &pr : b->preps) pr.moments = moments;
+    b->P.late_unbounded = 1;          /* every run is admitted before the first round */
     if (!rc) rc = pool_create(&b->P, cls, nruns, total, per_particle, tune);
     if (!rc && !tune) {
         std::string key = class_key(cls, total, nruns) + (moments ? "" : "/density");

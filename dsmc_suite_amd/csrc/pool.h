@@ -79,6 +79,11 @@ struct Pool {
 
     dsmc_launch_shape shape;
     int graph_rounds = 8, use_graph = 1;
+    /* batches (every run admitted before the first round): the rounds after the
+     * first of each replay run without a step budget, so the particles paused by
+     * the first round fly to their ends in one launch instead of one launch per
+     * budget, each with its own tail of a few long particles */
+    int late_unbounded = 0;
     hipGraphExec_t exec = nullptr;
     int64_t rounds = 0;
     int64_t inflight = 0;                      /* particles of runs not done */

@@ -143,14 +143,14 @@ static int launch_rounds(Pool *P, hipStream_t s)
         int p = r & 1;
         if (P->hist_bins) {        /* collisional particles: their own queues and kernel */
             HIP_OK(dsmc_launch_phys(nullptr, nullptr, &D1, D1.rec, Q1, p, fgrid, P->shape.block,
-                                    P->shape.relaunch_batch, P->shape.step_budget, s));
+                                    P->shape.relaunch_batch, P->late_unbounded && r > 0 ? 0 : P->shape.step_budget, s));
         }
+        const int64_t budget = P->late_unbounded && r > 0 ? 0 : P->shape.step_budget;
         if (P->shape.service_inline) {
             HIP_OK(dsmc_launch_phys(nullptr, nullptr, &D, D.rec, Q, p, fgrid, P->shape.block,
-                                    P->shape.relaunch_batch, P->shape.step_budget, s));
+                                    P->shape.relaunch_batch, budget, s));
         } else {
-            HIP_OK(dsmc_launch_pool_flight(&D, Q, p, fgrid, P->shape.block, P->shape.relaunch_batch,
-                                           P->shape.step_budget, s));
+            HIP_OK(dsmc_launch_pool_flight(&D, Q, p, fgrid, P->shape.block, P->shape.relaunch_batch, budget, s));
             HIP_OK(dsmc_launch_pool_service(&D, Q, p, sgrid, s));
         }
     }
