@@ -229,7 +229,9 @@ typedef struct dsmc_launch_shape {
     int relaunch_batch;       /* split kernels: idle lanes a wavefront gathers before pulling
                                  new particles; service inline: lanes whose hop ended that a
                                  wavefront gathers before running the landing service */
-    int step_budget;          /* steps a particle takes per flight launch before yielding */
+    int step_budget;          /* steps a particle takes per flight launch before yielding (batches:
+                                 the first round only; their later rounds run without a budget,
+                                 every run being admitted before the first) */
     int service_inline;       /* 1 = landings serviced inside the flight kernel (physical and
                                  compat ballistic runs), 0 = by a separate service kernel per
                                  round (physical advanced runs are always serviced inline,
