@@ -412,6 +412,30 @@ extern "C" int emu_locate(const dsmc_params *p, const dsmc_surface *s, int64_t n
     return 0;
 }
 
+/* Face location of n directions as the pool kernels make it: no guess, the
+ * first four candidates' normals read from the face-map cell (fmap_n). */
+extern "C" int emu_locate_pool(const dsmc_params *p, const dsmc_surface *s, int64_t n, const double *dirs,
+                               int32_t *out)
+{
+    if (p->mode != DSMC_MODE_PHYSICAL) return -1;
+    DsmcEnv E;
+    dsmc::HostTables T;
+    int rc = dsmc::build_env(p, s, 0, &E, &T);
+    if (rc) return rc;
+    E.enorm = T.enorm.data();
+    E.fnorm = T.fnorm.data();
+    attach_face_map(E, T);
+    std::vector<double> fmn;
+    dsmc::build_face_map_normals(T.fmap, T.fnorm, E.fmap_g, fmn);
+    E.fmap_n = fmn.data();
+    for (int64_t i = 0; i < n; i++) {
+        const double *d = dirs + 3 * i;
+        const double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        out[i] = locate_face_near<true>(E, -1, d[0], d[1], d[2], len);
+    }
+    return 0;
+}
+
 /* Which step of locate_face_near resolves each of n directions (no hint):
  * counts[0] the first four face-map candidates, [1] the next four, [2] the
  * hierarchical search (no candidate holds the direction firmly). */

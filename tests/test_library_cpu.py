@@ -200,6 +200,12 @@ def test_face_map_locates_as_the_full_search(lib, emu, levels):
         bad = np.nonzero(near != full)[0]
         assert len(bad) == 0, (len(bad), dirs[bad[:5]], near[bad[:5]], full[bad[:5]])
     assert len(np.unique(full)) == nfine
+    # the pools' search: the first candidates' normals stored in the face-map cell
+    pool = np.zeros(n, np.int32)
+    emu.emu_locate_pool.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+    assert emu.emu_locate_pool(C.byref(p), C.byref(surf.surface), n, dirs.ctypes.data, pool.ctypes.data) == 0
+    bad = np.nonzero(pool != full)[0]
+    assert len(bad) == 0, (len(bad), dirs[bad[:5]], pool[bad[:5]], full[bad[:5]])
 
 
 @pytest.mark.parametrize("levels", [3, 4])
