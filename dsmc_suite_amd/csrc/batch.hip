@@ -147,6 +147,10 @@ static int tune_pool(Pool *P, const std::vector<PoolPrep> &preps, double frac, d
         pr.E.nparticles = n;
     }
     auto trial = [&](const dsmc_launch_shape &s, double *score) -> int {
+        /* the flight kernel is persistent: a grid of more blocks than the CUs
+         * hold leaves the extra blocks to start as others end, a shape that
+         * won on scaled-down samples but ran the 64-run batch 1-2% slower */
+        if (s.blocks_per_cu > pool_occupancy(P, s.block)) { *score = -1.0; return DSMC_OK; }
         int rc = pool_set_shape(P, s);
         if (rc) return rc;
         std::vector<int> gs;
@@ -241,7 +245,9 @@ extern "C" dsmc_batch *dsmc_batch_create(int nruns, const dsmc_params *params, c
         }
         if (hit) rc = pool_set_shape(&b->P, s);
         else {
-            /* a copy large enough to fill the device: >= 4M particles or the whole batch */
+            /* a copy large enough to fill the device: >= 4M particles or the whole
+             * batch (a 12M-particle copy of the 64-run batch took twice as long to
+             * tune and chose the same shape) */
             double frac = std::min(1.0, std::max(1.0 / 16.0, 4.0e6 / (double)total));
             rc = tune_pool(&b->P, b->preps, frac, &s);
             if (!rc) {

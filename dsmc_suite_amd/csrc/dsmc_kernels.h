@@ -41,6 +41,7 @@ hipError_t dsmc_launch_flight(const DsmcEnv *E, DsmcRec *rec, DsmcQueues Q, int 
 hipError_t dsmc_launch_handover(DsmcQueues Q, int p, int grid, hipStream_t st);
 /* service kernel of round p (one run), then the queue reset */
 extern "C" int dsmc_flight_occupancy(const DsmcEnv *E, int inline_svc, int block);
+extern "C" int dsmc_pool_occupancy(const DsmcPoolDev *P, int block);
 hipError_t dsmc_launch_service(const DsmcEnv *E, DsmcRec *rec, DsmcQueues Q, int p, int grid, hipStream_t st);
 /* the same two for a pool of runs */
 hipError_t dsmc_launch_pool_flight(const DsmcPoolDev *P, DsmcQueues Q, int p, int grid, int block,

@@ -1171,6 +1171,22 @@ extern "C" int dsmc_flight_occupancy(const DsmcEnv *E, int inline_svc, int block
     return e == hipSuccess ? nb : 0;
 }
 
+/* Blocks of `block` threads of a pool's flight kernel with the landing service
+ * inline that one CU holds at once (its LDS as launched), 0 on error. */
+extern "C" int dsmc_pool_occupancy(const DsmcPoolDev *P, int block)
+{
+    if (!block_ok(block)) return 0;
+    const size_t lds = sizeof(double) * (size_t)((P->ftd + 1) + 2 * (P->ffd + 1) + P->nhr);
+    int nb = 0;
+    hipError_t e;
+    if (P->adv) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, phys_kernel<true, 1, true>, block, lds);
+    else if (P->steps_lds > 0)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, phys_kernel<true, 0, true, true>, block,
+                                                         lds + (size_t)P->steps_lds * sizeof(DsmcStepEnv));
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, phys_kernel<true, 0, true>, block, lds);
+    return e == hipSuccess ? nb : 0;
+}
+
 extern "C" hipError_t dsmc_launch_service(const DsmcEnv *E, DsmcRec *rec, DsmcQueues Q, int p, int grid,
                                           hipStream_t st)
 {

@@ -137,6 +137,8 @@ int pool_collect(Pool *P, int g, dsmc_result *r);
  * released: its particles hold their slots until they finish). */
 int pool_release(Pool *P, int g);
 int pool_set_shape(Pool *P, const dsmc_launch_shape &s);
+/* Blocks of `block` threads of the pool's flight kernel one CU holds at once. */
+int pool_occupancy(Pool *P, int block);
 
 }  // namespace dsmc
 

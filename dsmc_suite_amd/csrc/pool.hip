@@ -129,6 +129,12 @@ static DsmcPoolDev pool_dev(Pool *P)
     return D;
 }
 
+int pool_occupancy(Pool *P, int block)
+{
+    const DsmcPoolDev D = pool_dev(P);
+    return dsmc_pool_occupancy(&D, block);
+}
+
 /* One round sequence of graph_rounds rounds on stream s. */
 static int launch_rounds(Pool *P, hipStream_t s)
 {
